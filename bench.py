@@ -1,0 +1,163 @@
+"""Throughput of the engine_step hot path on one MI355X.
+
+Workload: the 2,000,000-particle Morse+LJ system (4 types, density 1,
+cutoff 2.5, tabulated potentials), FP32-mixed path.  One step = binning,
+nonbonded pass, bonded pass, forward Euler with migration, all on the GPU
+with the particles resident in HBM.
+
+Prints one JSON line with particle-steps/s, the FP64 GPU path beside it,
+the force error of both paths against the FP64 CPU oracle, the roofline of
+the nonbonded kernel (HIP events on the module's stream) and the CPU
+baseline (the oracle restatement of the reference CPU engine, timed on a
+bounded number of steps on this host's cores).
+"""
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "tissue-forge_amd")
+
+# HBM3E peak of one MI355X; dense FP32 vector peak without packed math
+HBM_PEAK_GBS = 8000.0
+# Tile-streamed bytes per particle-step of the nonbonded kernel: every
+# particle's 16-byte position record is read by the 27 cells of its stencil,
+# plus its own force write (27*16 + 16).
+NONBOND_BYTES_PER_PARTICLE = 448.0
+
+
+def load_package():
+    spec = importlib.util.spec_from_file_location("tissue_forge_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["tissue_forge_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def force_errors(fg, fo):
+    d = np.linalg.norm(fg[:, :3] - fo[:, :3], axis=1)
+    mag = np.linalg.norm(fo[:, :3], axis=1)
+    scale = math.sqrt(float(np.mean(mag ** 2)))
+    rel = d / np.maximum(mag, 1e-3 * scale)
+    return dict(err_rms=float(d.max() / scale), rel_p99=float(np.percentile(rel, 99)), rel_max=float(rel.max()))
+
+
+def time_gpu(tf, W, scene, pots, precision, steps, warmup):
+    eng, _ = W.build_engine(scene, potentials=pots)
+    eng.to_device(precision)
+    eng.force()
+    forces = eng.particles()["f"]
+    eng.step(max(warmup, 1))
+    eng.set_profiling(True)
+    eng.step(steps)   # device timing of the timed region (events around each phase)
+    kt = eng.kernel_times()
+    eng.set_profiling(False)
+    t0 = time.perf_counter()
+    eng.step(steps)
+    wall = time.perf_counter() - t0
+    dev_ms = eng.last_step_ms()
+    eng.from_device()
+    eng.close()
+    return dict(wall_s=wall, dev_ms=dev_ms, kernels=kt, forces=forces)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=2_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has a single MI355X, multi-GPU runs are not part "
+              "of this build", file=sys.stderr)
+        sys.exit(2)
+
+    tf = load_package()
+    from tissue_forge_amd import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    n = args.n
+    scene = W.morse_lj(n=n)
+    pots = [W.make_potential(p) for p in scene["pots"]]
+    log("workload morse_lj n=%d L=%.3f cells=%s" % (n, scene["dim"][0], scene["cells"]))
+
+    mixed = time_gpu(tf, W, scene, pots, tf.PREC_MIXED, args.steps, args.warmup)
+    log("mixed: wall %.3f s for %d steps, device %.3f ms, kernels %s" %
+        (mixed["wall_s"], args.steps, mixed["dev_ms"], mixed["kernels"]))
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = time_gpu(tf, W, scene, pots, tf.PREC_FP64, args.steps, args.warmup)
+        log("fp64: wall %.3f s, kernels %s" % (fp64["wall_s"], fp64["kernels"]))
+
+    value = n * args.steps / mixed["wall_s"]
+    ms_per_step = mixed["wall_s"] / args.steps * 1e3
+    nb_ms = mixed["kernels"]["nonbond_avg"]
+    achieved = NONBOND_BYTES_PER_PARTICLE * n / (nb_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "particle-steps/s at 1 MI355X (FP32-mixed vs FP64, force err <=1e-5); HBM roofline %",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-mixed (fp32 pair/integrator arithmetic, fp64 reductions)",
+        "data": "synthetic (seeded uniform positions)",
+        "config": {"workload": "morse_lj_2M: %d particles, 4 types, density 1, cutoff 2.5, Morse like / LJ unlike, "
+                               "cells %s" % (n, scene["cells"]),
+                   "parallelism": "single GPU"},
+        "phases_ms": {k: mixed["kernels"][k] for k in ("bin_avg", "nonbond_avg", "advance_avg")},
+        "roofline": {"bound": "hbm", "kernel": "k_nonbond", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_particle": NONBOND_BYTES_PER_PARTICLE},
+    }
+    if fp64 is not None:
+        v64 = n * args.steps / fp64["wall_s"]
+        out["fp64_path"] = {"value": v64, "ms_per_step": fp64["wall_s"] / args.steps * 1e3,
+                            "phases_ms": {k: fp64["kernels"][k] for k in ("bin_avg", "nonbond_avg", "advance_avg")},
+                            "mixed_over_fp64": value / v64}
+
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        orc = O.Oracle(scene, pots)
+        orc.force(threads)
+        fo = orc.particles()["f"]
+        out["force_error_vs_fp64_oracle"] = {"mixed": force_errors(mixed["forces"], fo)}
+        if fp64 is not None:
+            out["force_error_vs_fp64_oracle"]["fp64_gpu"] = force_errors(fp64["forces"], fo)
+        log("oracle setup+force %.1f s" % (time.perf_counter() - t0))
+        t0 = time.perf_counter()
+        orc.step(args.cpu_steps, threads)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * args.cpu_steps / cpu_s, "unit": "particle-steps/s", "cores": threads,
+                               "kind": "port",
+                               "sample": "%d full steps of the same %d-particle system (FP64 restatement of the "
+                                         "reference CPU engine: cell tasks, Newton-3, sorted sweeps)" %
+                                         (args.cpu_steps, n)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

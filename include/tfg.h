@@ -1,0 +1,179 @@
+/*
+ * tfg.h -- C ABI of the MI355X engine_step module (libtfg.so).
+ *
+ * Plain C types only.  Every function returns an HRESULT-style code
+ * (0 = success, negative = failure; reference include/tf_port.h:255-282)
+ * unless stated otherwise, and leaves a message for tfg_last_error()
+ * (the role of the reference's error stack, source/tfError.h:78).
+ *
+ * Each declaration names the reference interface whose role it takes
+ * (paths relative to the reference repository).
+ */
+#ifndef TFG_H
+#define TFG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tfg_engine tfg_engine;
+typedef struct tfg_potential tfg_potential;
+typedef struct tfg_ensemble tfg_ensemble;
+
+#define TFG_OK 0
+#define TFG_E_FAIL ((int)0x80004005)
+#define TFG_E_INVALIDARG ((int)0x80070057)
+#define TFG_E_OUTOFMEMORY ((int)0x8007000E)
+#define TFG_E_RANGE ((int)0x8000000B)
+#define TFG_E_STATE ((int)0x8000FFFF)
+#define TFG_E_DEVICE ((int)0x80004006)
+#define TFG_E_TOOFAST ((int)0x80004007)
+
+/* Precision of the device path. */
+#define TFG_PREC_MIXED 0 /* FP32 pair/bonded/integrator arithmetic, FP64 reductions */
+#define TFG_PREC_FP64 1  /* all FP64 */
+
+/* Dynamics (source/mdcore/include/tfParticle.h:63-64). */
+#define TFG_NEWTONIAN 0
+#define TFG_OVERDAMPED 1
+/* Flux kinds (source/mdcore/include/tfFlux.h:40-42). */
+#define TFG_FLUX_FICK 0
+#define TFG_FLUX_SECRETE 1
+#define TFG_FLUX_UPTAKE 2
+/* DPD random term: unit (the reference's sigma*omega/sqrt(dt),
+ * source/mdcore/src/tf_dpd_eval.h:64) or a Philox4x32-10 draw per pair and step. */
+#define TFG_DPD_NOISE_UNIT 0
+#define TFG_DPD_NOISE_PHILOX 1
+
+/* Last error message of the calling thread (source/tfError.h:78). */
+const char *tfg_last_error(void);
+const char *tfg_version(void);
+
+/* ---- engine lifecycle: engine_init (source/mdcore/src/tfEngine.cpp:1702-1874)
+ *      as called by universe_init (source/tfSimulator.cpp:629-666). ---- */
+int tfg_engine_create(tfg_engine **out, const double origin[3], const double dim[3],
+                      const int cells[3], double cutoff, int max_types, double dt,
+                      int nr_fluxsteps);
+int tfg_engine_destroy(tfg_engine *e);
+/* space_init geometry (source/mdcore/src/tfSpace.cpp:737-770, 893-895). */
+int tfg_engine_geometry(const tfg_engine *e, int cdim[3], double h[3], int span[3]);
+/* Universe max_distance -> particle_max_dist_fraction (source/tfSimulator.cpp:661-666). */
+int tfg_engine_set_max_distance_fraction(tfg_engine *e, double fraction);
+/* Species count per particle (the state vector of source/state/, flat here). */
+int tfg_engine_set_nspecies(tfg_engine *e, int nspecies);
+int tfg_engine_set_dpd_noise(tfg_engine *e, int mode, uint64_t seed);
+int64_t tfg_engine_time(const tfg_engine *e);
+
+/* ---- particle types: engine_addtype (source/mdcore/src/tfEngine.cpp:1040-1052).
+ *      Returns the new type id (>= 0) or an error. ---- */
+int tfg_engine_addtype(tfg_engine *e, const char *name, double mass, double radius, int dynamics);
+
+/* ---- potentials: Potential::morse / lennard_jones_12_6 / harmonic /
+ *      harmonic_angle / dpd (source/mdcore/src/tfPotential.cpp:2741-2751,
+ *      2961-2975, 2991-3003, 3068-3084, 3128-3142).  Tables are built by
+ *      potential_init (:1396-1634).  Negative tol selects the reference default. ---- */
+int tfg_potential_morse(tfg_potential **out, double d, double a, double r0, double min,
+                        double max, double tol, int shifted);
+int tfg_potential_lennard_jones_12_6(tfg_potential **out, double min, double max, double A,
+                                     double B, double tol);
+int tfg_potential_harmonic(tfg_potential **out, double k, double r0, double min, double max,
+                           double tol);
+int tfg_potential_harmonic_angle(tfg_potential **out, double k, double theta0, double min,
+                                 double max, double tol);
+int tfg_potential_dpd(tfg_potential **out, double alpha, double gamma, double sigma,
+                      double cutoff, int shifted);
+int tfg_potential_destroy(tfg_potential *p);
+/* Table description: kind, flags, n, a, b, r0_plusone, alpha[0..2]. */
+int tfg_potential_info(const tfg_potential *p, int *kind, uint32_t *flags, int *n,
+                       double *a, double *b, double *r0_plusone, double alpha[3]);
+/* Copies up to cap coefficients (8*(n+1) in total); returns the count. */
+int tfg_potential_coeffs(const tfg_potential *p, double *out, int cap);
+/* Potential::operator() / force at distance r (potential_eval_ex,
+ * source/mdcore/src/tf_potential_eval.h:104-146): energy and force/r. */
+int tfg_potential_eval(const tfg_potential *p, double r, double ri, double rj, double *e,
+                       double *f_over_r);
+
+/* ---- bindings: engine_addpot (source/mdcore/src/tfEngine.cpp:1054-1077). ---- */
+int tfg_engine_addpot(tfg_engine *e, const tfg_potential *p, int type_i, int type_j);
+/* Fluxes::create (source/mdcore/src/tfFlux.cpp:56-112); cutoff <= 0 means global. */
+int tfg_engine_addflux(tfg_engine *e, int kind, int type_a, int type_b, int species_a,
+                       int species_b, double k, double decay, double target, double cutoff);
+
+/* ---- particles: space_addpart / space_setpartp (source/mdcore/src/tfSpace.cpp:432-475).
+ *      x, v: n*3 global coordinates; species: n*nspecies or NULL.
+ *      ids_out receives the new particle ids. ---- */
+int tfg_engine_addparts(tfg_engine *e, int n, const int *types, const double *x,
+                        const double *v, const double *species, int32_t *ids_out);
+int tfg_engine_set_particle_flags(tfg_engine *e, int n, const int32_t *ids, const uint32_t *flags);
+int tfg_engine_nparts(const tfg_engine *e);
+
+/* ---- bonded topology: Bond::create / engine_bond_alloc
+ *      (source/mdcore/src/tfBond.cpp:636-654, tf_engine_bonded.cpp:1140) and
+ *      Angle::create / engine_angle_alloc (tf_engine_bonded.cpp:975). ---- */
+int tfg_engine_addbonds(tfg_engine *e, const tfg_potential *p, int n, const int32_t *ij);
+int tfg_engine_addangles(tfg_engine *e, const tfg_potential *p, int n, const int32_t *ijk);
+
+/* ---- device toggle: cuda::engine_toCUDA / engine_fromCUDA
+ *      (source/mdcore/src/tfEngine.cpp:1136-1197) as wrapped by
+ *      cuda::EngineConfig::toDevice / fromDevice / setDevice
+ *      (source/cuda/tfEngineConfig.cpp:67-172). ---- */
+int tfg_device_count(void);
+int tfg_device_set(int ordinal);
+/* Uploads the engine; a second call while on the device is a no-op. */
+int tfg_engine_to_device(tfg_engine *e, int precision);
+/* Downloads x, v, f, cell ids and species, frees the device copy. */
+int tfg_engine_from_device(tfg_engine *e);
+int tfg_engine_on_device(const tfg_engine *e);
+/* cuda::EngineConfig::setBlocks / setThreads (source/cuda/tfEngineConfig.cpp:123-160):
+ * workgroups of the pair kernel, threads of the streaming kernels (0 = keep). */
+int tfg_set_launch(tfg_engine *e, int blocks, int threads);
+
+/* ---- step: engine_step (source/mdcore/src/tfEngine.cpp:1378-1429), whole step on
+ *      the device (binning, nonbonded, bonded, flux, forward Euler). ---- */
+int tfg_engine_step(tfg_engine *e, int nsteps);
+/* engine_force only (source/mdcore/src/tfEngine.cpp:1504-1557): binning, nonbonded
+ * and bonded forces, no update.  The role of cuda::engine_nonbond_cuda
+ * (source/mdcore/src/tfRunner_cuda.cu:2940-3354) plus the bonded pass. */
+int tfg_nonbond_eval(tfg_engine *e);
+
+/* ---- readback (synchronous). ---- */
+/* out[0..5] = epot, epot_nonbond, epot_bond, epot_angle, ekin, virial of the
+ * last evaluated step (space epot fields, source/mdcore/include/tfSpace.h:207;
+ * engine_kinetic_energy, source/mdcore/src/tfEngine.cpp:1893-1927). */
+int tfg_read_energies(tfg_engine *e, double out[6]);
+/* Per particle, indexed by id (any pointer may be NULL): cell id, position
+ * relative to the cell origin (n*3), velocity (n*3), force and number
+ * density (n*4), species (n*nspecies), pending flux (n*nspecies).  Downloads
+ * from the device without leaving it. */
+int tfg_read_particles(tfg_engine *e, int32_t *cid, double *x, double *v, double *f,
+                       double *species, double *species_q);
+/* Global positions origin + loc*h + x (n*3). */
+int tfg_read_positions(tfg_engine *e, double *xg);
+/* FP64 per-species totals (flux conservation); returns the count. */
+int tfg_read_species_totals(tfg_engine *e, double *out, int cap);
+/* Device time of the last tfg_engine_step call in ms, and the per-kernel split
+ * (bin, nonbond, advance) of its last step; returns the number of entries. */
+double tfg_last_step_ms(const tfg_engine *e);
+/* Per-phase device time: [bin, nonbond, advance] of the last step, then the
+ * averages over all steps of the last call when profiling is on. */
+int tfg_kernel_times(const tfg_engine *e, double *ms, int cap);
+/* Records HIP events around every phase of every step (the role of the
+ * reference's engine timers, source/mdcore/include/tfEngine.h:95-145). */
+int tfg_set_profiling(tfg_engine *e, int on);
+
+/* ---- batched ensemble: independent systems advanced by the same launches
+ *      (no reference counterpart: the reference engine is a process-global
+ *      singleton, source/mdcore/include/tfEngine.h:1390). ---- */
+int tfg_ensemble_create(tfg_ensemble **out, tfg_engine *const *engines, int n, int precision);
+int tfg_ensemble_step(tfg_ensemble *ens, int nsteps, int use_graph);
+int tfg_ensemble_destroy(tfg_ensemble *ens, int download);
+double tfg_ensemble_last_ms(const tfg_ensemble *ens);
+int tfg_ensemble_set_launch(tfg_ensemble *ens, int blocks, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TFG_H */

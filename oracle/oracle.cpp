@@ -1,0 +1,1010 @@
+// ORACLE -- test infrastructure only.
+//
+// FP64 CPU restatement of the reference engine_step (Tissue Forge 0.2.1,
+// mdcore with FPTYPE = double), used by tests/ as the parity checker and by
+// bench.py as the timed CPU baseline.  Nothing in the product links,
+// loads or calls this file.
+//
+// Restated functions (reference paths relative to source/mdcore/):
+//   engine_step / force_prep / force ...... src/tfEngine.cpp:1378-1557
+//   engine_kinetic_energy ................. src/tfEngine.cpp:1893-1927
+//   space_init geometry + task list ....... src/tfSpace.cpp:715-1062
+//   space_getsid, cell_flip/sortlistID .... src/tfSpace.cpp:75-114, src/tfSpace_cell.cpp:54-99
+//   space_setpartp ........................ src/tfSpace.cpp:432-475
+//   space_prepare ......................... src/tfSpace.cpp:168-208
+//   space_cell_add / welcome .............. src/tfSpace_cell.cpp:187-319
+//   runner_dosort + sort_descending ....... src/tf_runner_dosort.cpp:64-126, src/tfRunner.cpp:73-136
+//   runner_dopair / runner_doself ......... src/tf_runner_dopair.cpp:72-332, 514-758
+//   potential_eval_ex / _super_ex ......... src/tf_potential_eval.h:43-146, 207-272
+//   potential_eval / potential_eval_r ..... src/tf_potential_eval.h:74-102, 165-194
+//   dpd_eval .............................. src/tf_dpd_eval.h:32-76
+//   flux_eval_ex .......................... src/tf_flux_eval.h:32-103
+//   Fluxes_integrate ...................... src/tfFlux.cpp:175-202
+//   w_cubic_spline ........................ src/tf_smoothing_kernel.h:34-59
+//   bond_eval / angle_eval ................ src/tfBond.cpp:97-345, src/tfAngle.cpp:90-384
+//   engine_advance_forward_euler .......... src/tf_engine_advance.cpp:122-247, 309-480
+//
+// Decisions where the reference is nondeterministic or undefined:
+//   * W() uses an exact sqrt (the reference's x86 qsqrt is a 12-bit estimate).
+//   * Degenerate angles (cos = 0 or -1) use the regular derivative formula
+//     instead of a random direction.
+//   * Between particles of equal type, the flux "a" role goes to the lower id
+//     (the reference uses thread-dependent storage order).
+//   * The DPD random term is 1 (the reference's mode) or a Philox4x32-10
+//     draw keyed by (min id, max id, step) and the seed.
+//   * Persistent forces (force_i) are zero; no clusters, no ghosts.
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr int kMaxS = 8;
+
+enum { POT_SCALED = 1u << 11, POT_SHIFTED = 1u << 12 };
+enum { KIND_TABLE = 0, KIND_DPD = 1 };
+enum { F_FROZEN_X = 1u << 3, F_FROZEN_Y = 1u << 4, F_FROZEN_Z = 1u << 5, F_CLUSTER = 1u << 1 };
+
+struct Pot {
+    int kind = KIND_TABLE;
+    unsigned flags = 0;
+    double a = 0, b = 0, r0p1 = 1;
+    double alpha[3] = {0, 0, 0};
+    int n = 0;
+    std::vector<double> c;
+    double dalpha = 0, dgamma = 0, dsigma = 0;
+};
+
+struct Flux {
+    int kind, ta, tb, ia, ib;
+    double coef, decay, target, cutoff;
+};
+
+struct Part {
+    int id = 0, type = 0;
+    unsigned flags = 0;
+    double x[3] = {0, 0, 0}, v[3] = {0, 0, 0}, f[4] = {0, 0, 0, 0};
+    double imass = 1, radius = 1;
+    double s[kMaxS] = {0}, q[kMaxS] = {0};
+};
+
+struct Cell {
+    int loc[3];
+    double origin[3];
+    std::vector<Part> parts, incoming;
+    std::vector<unsigned> sortlist;   // 13 * count
+    double epot = 0;
+    std::mutex mtx;
+};
+
+struct Task {
+    int type;   // 0 self, 1 pair
+    int i, j, sid;
+};
+
+struct Type {
+    double mass, imass, radius;
+    int dynamics;
+};
+
+// Unit directions of the 13 sort lists (src/tfSpace_cell.cpp:81-95).
+double g_cell_shift[13 * 3];
+const char g_sortlist_id[27] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 0,
+                                12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0};
+struct ShiftInit {
+    ShiftInit() {
+        int n = 0;
+        // Directions with the first non-zero component positive, ordered as
+        // the 13 "flipped" shifts (1,1,1), (1,1,0), ... (0,0,1).
+        const int dirs[13][3] = {{1, 1, 1}, {1, 1, 0}, {1, 1, -1}, {1, 0, 1}, {1, 0, 0}, {1, 0, -1}, {1, -1, 1},
+                                 {1, -1, 0}, {1, -1, -1}, {0, 1, 1}, {0, 1, 0}, {0, 1, -1}, {0, 0, 1}};
+        for (auto &d : dirs) {
+            const double nrm = std::sqrt(double(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+            for (int k = 0; k < 3; k++) g_cell_shift[3 * n + k] = d[k] / nrm;
+            n++;
+        }
+    }
+} g_shift_init;
+
+void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; r++) {
+        const uint64_t a = 0xD2511F53ull * c[0];
+        const uint64_t b = 0xCD9E8D57ull * c[2];
+        const uint32_t y0 = uint32_t(b >> 32) ^ c[1] ^ k0;
+        const uint32_t y1 = uint32_t(b);
+        const uint32_t y2 = uint32_t(a >> 32) ^ c[3] ^ k1;
+        const uint32_t y3 = uint32_t(a);
+        c[0] = y0; c[1] = y1; c[2] = y2; c[3] = y3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+double xi_of(uint32_t ida, uint32_t idb, uint64_t step, uint64_t seed) {
+    uint32_t c[4] = {std::min(ida, idb), std::max(ida, idb), uint32_t(step), uint32_t(step >> 32)};
+    philox(c, uint32_t(seed), uint32_t(seed >> 32));
+    const int m = int(c[0] >> 9) - 4194304;
+    return (double(m) + 0.5) * 2.384185791015625e-07 * 1.7320508075688772;
+}
+
+struct System {
+    double origin[3], dim[3], h[3], ih[3], cutoff, cutoff2;
+    int cdim[3], span[3], ncells;
+    double dt, dt_flux;
+    double maxdx = 0.0;      // space->maxdx (0 without Verlet lists)
+    double max_frac = 0.05;  // particle_max_dist_fraction
+    int nspecies = 0;
+    int max_types;
+    std::vector<Type> types;
+    std::vector<Pot> pots;
+    std::vector<int> potmap;
+    std::vector<Flux> fluxes;
+    std::vector<std::vector<int>> fluxmap;   // per type pair: flux entry ids
+    std::vector<Cell> cells;
+    std::vector<Task> tasks;
+    std::vector<std::pair<int, int>> where;   // id -> (cell, index)
+    std::vector<int> bonds;    // i, j, pot
+    std::vector<int> angles;   // i, j, k, pot
+    int nparts = 0;
+    int64_t time = 0;
+    int dpd_noise = 0;
+    uint64_t seed = 0;
+    // 1: evaluate the sweep bounds in unsigned arithmetic exactly as
+    // tf_runner_dopair.cpp:151,163 do (a negative projected gap wraps and
+    // ends the row); 0: the signed comparison the sweep is meant to make.
+    int sweep_unsigned = 0;
+    double epot = 0, epot_nb = 0, epot_bond = 0, epot_angle = 0, ekin = 0, virial = 0;
+    std::atomic<int> toofast{0};
+
+    int cellid(int i, int j, int k) const { return (i * cdim[1] + j) * cdim[2] + k; }
+
+    void reindex() {
+        where.assign(nparts, {-1, -1});
+        for (int c = 0; c < ncells; c++)
+            for (int p = 0; p < (int)cells[c].parts.size(); p++) where[cells[c].parts[p].id] = {c, p};
+    }
+    Part &part(int id) { return cells[where[id].first].parts[where[id].second]; }
+};
+
+// space_getsid: minimum-image shift of cj relative to ci, flipped so the
+// first non-zero component is positive.
+int getsid(const System &s, int &ci, int &cj, double shift[3]) {
+    for (int k = 0; k < 3; k++) {
+        shift[k] = s.cells[cj].origin[k] - s.cells[ci].origin[k];
+        if (shift[k] * 2 > s.dim[k])
+            shift[k] -= s.dim[k];
+        else if (shift[k] * 2 < -s.dim[k])
+            shift[k] += s.dim[k];
+    }
+    int sid = 0;
+    for (int k = 0; k < 3; k++) sid = 3 * sid + ((shift[k] < 0) ? 0 : ((shift[k] > 0) ? 2 : 1));
+    if (sid < 13) {   // cell_flip
+        std::swap(ci, cj);
+        for (int k = 0; k < 3; k++) shift[k] = -shift[k];
+    }
+    return g_sortlist_id[sid];
+}
+
+double w_cubic(double r2, double hk) {
+    const double r = std::sqrt(r2);
+    const double x = r / hk;
+    double y;
+    if (x < 1.0) {
+        const double x2 = x * x;
+        y = 1.0 - 1.5 * x2 + 0.75 * x2 * x;
+    } else if (x < 2.0) {
+        const double a = 2.0 - x;
+        y = 0.25 * a * a * a;
+    } else {
+        y = 0.0;
+    }
+    return y / (M_PI * hk * hk * hk);
+}
+
+void horner(const double *c, double x, double &ee, double &eff) {
+    ee = c[2] * x + c[3];
+    eff = c[2];
+    for (int k = 4; k < 8; k++) {
+        eff = eff * x + ee;
+        ee = ee * x + c[k];
+    }
+}
+
+// potential_eval_ex (tf_potential_eval.h:104-146).
+bool eval_ex(const Pot &p, double ri, double rj, double r2, double &e, double &f) {
+    const double eps = std::numeric_limits<double>::epsilon();
+    double r = std::sqrt(r2);
+    const double ro = r < eps ? eps : r;
+    if (p.flags & POT_SCALED)
+        r = r / (ri + rj);
+    else if (p.flags & POT_SHIFTED)
+        r = r - (ri + rj) + p.r0p1;
+    r = r < p.a ? p.a : r;
+    const unsigned ind = unsigned(std::max(0.0, p.alpha[0] + r * (p.alpha[1] + r * p.alpha[2])));
+    if (r > p.b || ind > unsigned(p.n)) return false;
+    const double *c = &p.c[8 * ind];
+    double ee, eff;
+    horner(c, (r - c[0]) * c[1], ee, eff);
+    e = ee;
+    f = eff * c[1] / ro;
+    return true;
+}
+
+// potential_eval (tf_potential_eval.h:74-102): r from r2, no range check.
+void eval_plain(const Pot &p, double r2, double &e, double &f) {
+    const double r = std::sqrt(r2);
+    const int ind = int(std::fmax(0.0, p.alpha[0] + r * (p.alpha[1] + r * p.alpha[2])));
+    const double *c = &p.c[8 * ind];
+    double ee, eff;
+    horner(c, (r - c[0]) * c[1], ee, eff);
+    e = ee;
+    f = eff * c[1] / r;
+}
+
+// potential_eval_r (tf_potential_eval.h:165-194).
+bool eval_r(const Pot &p, double r, double &e, double &f) {
+    const int ind = int(std::fmax(0.0, p.alpha[0] + r * (p.alpha[1] + r * p.alpha[2])));
+    if (ind > p.n) return false;
+    const double *c = &p.c[8 * ind];
+    double ee, eff;
+    horner(c, (r - c[0]) * c[1], ee, eff);
+    e = ee;
+    f = eff * c[1];
+    return true;
+}
+
+// flux_eval_ex (tf_flux_eval.h:48-103) for one type pair.
+void flux_pair(const System &s, double r2, Part &p1, Part &p2) {
+    const std::vector<int> &lst = s.fluxmap[p1.type * s.max_types + p2.type];
+    const double r = std::sqrt(r2);
+    for (int fi : lst) {
+        const Flux &fl = s.fluxes[fi];
+        if (r > fl.cutoff) continue;
+        double term = 1 - r / fl.cutoff;
+        term = term * term;
+        Part *pi, *pj;
+        if (p1.type == p2.type) {
+            pi = p1.id < p2.id ? &p1 : &p2;
+            pj = p1.id < p2.id ? &p2 : &p1;
+        } else {
+            pi = p1.type == fl.ta ? &p1 : &p2;
+            pj = p2.type == fl.tb ? &p2 : &p1;
+        }
+        const double ssi = pi->s[fl.ia], ssj = pj->s[fl.ib];
+        double q = term;
+        if (fl.kind == 0) {
+            q *= fl.coef * (ssi - ssj);
+        } else if (fl.kind == 1) {
+            const double v = fl.coef * (ssi - fl.target);
+            q *= (v > 0.0) * v;
+        } else {
+            const double v = fl.coef * (fl.target - ssj) * ssi;
+            q *= (v > 0.0) * v;
+        }
+        const double hd = fl.decay / 2.0;
+        pi->q[fl.ia] = pi->q[fl.ia] - q - hd * ssi;
+        pj->q[fl.ib] = pj->q[fl.ib] + q - hd * ssj;
+    }
+}
+
+// One interacting pair (the body shared by runner_doself and runner_dopair):
+// number density, flux, then potential_eval_super_ex.
+inline void pair_body(const System &s, Part &pi, Part &pj, const double dx[3], double r2, double &epot,
+                      double &vir) {
+    const double nd = w_cubic(r2, s.cutoff);
+    pi.f[3] += nd;
+    pj.f[3] += nd;
+    const int tp = pi.type * s.max_types + pj.type;
+    const int pidx = s.potmap[tp];
+    const bool has_flux = !s.fluxmap[tp].empty();
+    if (pidx < 0 && !has_flux) return;
+    if (has_flux) flux_pair(s, r2, pi, pj);
+    if (pidx < 0) return;
+    const Pot &p = s.pots[pidx];
+    if (p.kind == KIND_DPD) {
+        // dpd_eval (tf_dpd_eval.h:32-76).
+        const double eps = std::numeric_limits<double>::epsilon();
+        const double delta = 1.0 / std::sqrt(s.dt);
+        double r = std::sqrt(r2);
+        const double ro = r < eps ? eps : r;
+        r = (p.flags & POT_SHIFTED) ? r - (pi.radius + pj.radius) : r;
+        if (r > p.b) return;
+        r = r >= p.a ? r : p.a;
+        const double e[3] = {dx[0] / ro, dx[1] / ro, dx[2] / ro};
+        const double v[3] = {pi.v[0] - pj.v[0], pi.v[1] - pj.v[1], pi.v[2] - pj.v[2]};
+        const double om = r < 0.0 ? 1.0 : (1 - r / p.b);
+        const double fc = p.dalpha * om;
+        const double fd = -p.dgamma * om * om * (e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
+        const double xi = s.dpd_noise ? xi_of(uint32_t(pi.id), uint32_t(pj.id), uint64_t(s.time), s.seed) : 1.0;
+        const double fr = p.dsigma * om * delta * xi;
+        const double f = fc + fd + fr;
+        for (int k = 0; k < 3; k++) {
+            pj.f[k] = pj.f[k] - f * e[k];
+            pi.f[k] = pi.f[k] + f * e[k];
+        }
+        vir += f * r2 / ro;
+        return;
+    }
+    double e, f;
+    if (!eval_ex(p, pi.radius, pj.radius, r2, e, f)) return;
+    for (int k = 0; k < 3; k++) {
+        const double w = f * dx[k];
+        pi.f[k] -= w;
+        pj.f[k] += w;
+    }
+    epot += e;
+    vir -= f * r2;
+}
+
+// runner_doself (tf_runner_dopair.cpp:514-758).
+void do_self(System &s, Cell &c, double &vir) {
+    double epot = 0.0;
+    const int n = (int)c.parts.size();
+    for (int i = 0; i < n; i++) {
+        Part &pi = c.parts[i];
+        for (int j = i + 1; j < n; j++) {
+            Part &pj = c.parts[j];
+            double dx[3];
+            for (int k = 0; k < 3; k++) dx[k] = pi.x[k] - pj.x[k];
+            const double r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+            if (r2 > s.cutoff2) continue;
+            pair_body(s, pi, pj, dx, r2, epot, vir);
+        }
+    }
+    c.epot += epot;
+}
+
+// runner_dosort (tf_runner_dosort.cpp:64-126): 13 projected keys per particle,
+// sorted in descending order of the low 16 bits.
+void do_sort(System &s, Cell &c) {
+    const int n = (int)c.parts.size();
+    c.sortlist.assign(13 * (size_t)n, 0u);
+    if (n == 0) return;
+    const double bias = std::sqrt(s.h[0] * s.h[0] + s.h[1] * s.h[1] + s.h[2] * s.h[2]);
+    const double dscale = double(SHRT_MAX) / (2 * bias);
+    for (int sid = 0; sid < 13; sid++) {
+        unsigned *ip = &c.sortlist[(size_t)n * sid];
+        const double *sh = &g_cell_shift[3 * sid];
+        for (int i = 0; i < n; i++) {
+            const Part &p = c.parts[i];
+            ip[i] = (unsigned(i) << 16) | unsigned(dscale * (bias + p.x[0] * sh[0] + p.x[1] * sh[1] + p.x[2] * sh[2]));
+        }
+        std::sort(ip, ip + n, [](unsigned a, unsigned b) { return (a & 0xffff) > (b & 0xffff); });
+    }
+}
+
+// runner_dopair (tf_runner_dopair.cpp:72-332) with the sorted sweep.
+void do_pair(System &s, int ci_id, int cj_id, double &vir) {
+    Cell *ci = &s.cells[ci_id], *cj = &s.cells[cj_id];
+    if (ci->parts.empty() || cj->parts.empty()) return;
+    double shift[3];
+    int a = ci_id, b = cj_id;
+    const int sid = getsid(s, a, b, shift);
+    ci = &s.cells[a];
+    cj = &s.cells[b];
+    const double bias = std::sqrt(s.h[0] * s.h[0] + s.h[1] * s.h[1] + s.h[2] * s.h[2]);
+    const double dscale = double(SHRT_MAX) / (2 * bias);
+    const int dmaxdist = int(2 + dscale * (s.cutoff + 2 * s.maxdx));
+    const double nshift = std::sqrt(shift[0] * shift[0] + shift[1] * shift[1] + shift[2] * shift[2]);
+    const int dnshift = int(dscale * nshift);
+    const int ni = (int)ci->parts.size(), nj = (int)cj->parts.size();
+    const unsigned *ip = &ci->sortlist[(size_t)ni * sid];
+    const unsigned *jp = &cj->sortlist[(size_t)nj * sid];
+    double epot = 0.0;
+    const bool uw = s.sweep_unsigned != 0;
+    auto gap_gt = [&](unsigned kj, unsigned ki) {
+        if (uw) return (kj & 0xffff) + unsigned(dnshift) - (ki & 0xffff) > unsigned(dmaxdist);
+        return int(kj & 0xffff) + dnshift - int(ki & 0xffff) > dmaxdist;
+    };
+    auto gap_lt = [&](unsigned kj, unsigned ki) {
+        if (uw) return (kj & 0xffff) + unsigned(dnshift) - (ki & 0xffff) < unsigned(dmaxdist);
+        return int(kj & 0xffff) + dnshift - int(ki & 0xffff) < dmaxdist;
+    };
+    for (int i = 0; i < ni; i++) {
+        if (gap_gt(jp[nj - 1], ip[i])) break;
+        Part &pi = ci->parts[ip[i] >> 16];
+        const double pix[3] = {pi.x[0] - shift[0], pi.x[1] - shift[1], pi.x[2] - shift[2]};
+        for (int j = nj - 1; j >= 0 && gap_lt(jp[j], ip[i]); j--) {
+            Part &pj = cj->parts[jp[j] >> 16];
+            double dx[3];
+            for (int k = 0; k < 3; k++) dx[k] = pix[k] - pj.x[k];
+            const double r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+            if (r2 > s.cutoff2) continue;
+            pair_body(s, pi, pj, dx, r2, epot, vir);
+        }
+    }
+    ci->epot += epot;
+}
+
+int build_space(System &s, const double origin[3], const double dim[3], const int cells[3], double cutoff) {
+    s.cutoff = cutoff;
+    s.cutoff2 = cutoff * cutoff;
+    s.ncells = 1;
+    for (int k = 0; k < 3; k++) {
+        s.origin[k] = origin[k];
+        s.dim[k] = dim[k];
+        const double L = (dim[k] - origin[k]) / cells[k];
+        s.cdim[k] = int(std::floor(dim[k] / L));
+        s.h[k] = s.dim[k] / s.cdim[k];
+        s.ih[k] = 1.0 / s.h[k];
+        s.span[k] = int(std::ceil(cutoff * s.ih[k]));
+        s.ncells *= s.cdim[k];
+    }
+    for (int k = 0; k < 3; k++)
+        if (s.cdim[k] < 2 * s.span[k] + 1) return -1;
+    s.cells = std::vector<Cell>(s.ncells);
+    int l[3];
+    for (l[0] = 0; l[0] < s.cdim[0]; l[0]++)
+        for (l[1] = 0; l[1] < s.cdim[1]; l[1]++)
+            for (l[2] = 0; l[2] < s.cdim[2]; l[2]++) {
+                Cell &c = s.cells[s.cellid(l[0], l[1], l[2])];
+                for (int k = 0; k < 3; k++) {
+                    c.loc[k] = l[k];
+                    c.origin[k] = origin[k] + l[k] * s.h[k];
+                }
+            }
+    // Task list (tfSpace.cpp:884-999), periodic in every direction.
+    s.tasks.clear();
+    for (int i = 0; i < s.cdim[0]; i++)
+        for (int j = 0; j < s.cdim[1]; j++)
+            for (int k = 0; k < s.cdim[2]; k++) {
+                const int id1 = s.cellid(i, j, k);
+                for (l[0] = -s.span[0]; l[0] <= s.span[0]; l[0]++) {
+                    const int ii = (i + l[0] + s.cdim[0]) % s.cdim[0];
+                    for (l[1] = -s.span[1]; l[1] <= s.span[1]; l[1]++) {
+                        const int jj = (j + l[1] + s.cdim[1]) % s.cdim[1];
+                        for (l[2] = -s.span[2]; l[2] <= s.span[2]; l[2]++) {
+                            double d2 = 0;
+                            for (int q = 0; q < 3; q++) {
+                                const double lh = s.h[q] * std::fmax(std::abs(l[q]) - 1, 0);
+                                d2 += lh * lh;
+                            }
+                            if (d2 > s.cutoff2) continue;
+                            const int kk = (k + l[2] + s.cdim[2]) % s.cdim[2];
+                            const int id2 = s.cellid(ii, jj, kk);
+                            if (id1 < id2 || (id1 == id2 && l[0] == 0 && l[1] == 0 && l[2] == 0)) {
+                                int a = id1, b = id2;
+                                double sh[3];
+                                int sid = 0;
+                                if (id1 != id2) sid = getsid(s, a, b, sh);
+                                s.tasks.push_back({id1 == id2 ? 0 : 1, a, b, sid});
+                            }
+                        }
+                    }
+                }
+            }
+    return 0;
+}
+
+// Nonbonded pass: sort tasks, then self/pair tasks.  With several threads,
+// a task runs only while both of its cells are locked (the taboo scheme of
+// tfRunner.cpp:204-380 / tfQueue.cpp:109-163).
+void nonbond(System &s, int nthreads) {
+    const double self_nd = w_cubic(0.0, s.cutoff);
+    for (Cell &c : s.cells) {
+        c.epot = 0.0;
+        for (Part &p : c.parts) {
+            p.f[0] = p.f[1] = p.f[2] = 0.0;
+            p.f[3] = self_nd;
+        }
+    }
+    double vir_total = 0.0;
+    if (nthreads <= 1) {
+        for (Cell &c : s.cells) do_sort(s, c);
+        for (const Task &t : s.tasks) {
+            if (t.type == 0)
+                do_self(s, s.cells[t.i], vir_total);
+            else
+                do_pair(s, t.i, t.j, vir_total);
+        }
+        s.virial = vir_total;
+        return;
+    }
+    {
+        std::vector<std::thread> th;
+        std::atomic<int> next{0};
+        for (int t = 0; t < nthreads; t++)
+            th.emplace_back([&]() {
+                for (int c = next++; c < s.ncells; c = next++) do_sort(s, s.cells[c]);
+            });
+        for (auto &t : th) t.join();
+    }
+    const int nt = (int)s.tasks.size();
+    std::vector<std::atomic<char>> taboo(s.ncells);
+    for (auto &t : taboo) t.store(0);
+    std::vector<std::atomic<char>> done(nt);
+    for (auto &d : done) d.store(0);
+    std::atomic<int> first{0}, remaining{nt};
+    std::vector<double> vir(nthreads, 0.0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++)
+        th.emplace_back([&, t]() {
+            while (remaining.load() > 0) {
+                bool got = false;
+                for (int k = first.load(); k < nt; k++) {
+                    if (done[k].load()) {
+                        if (k == first.load()) first.compare_exchange_strong(k, k + 1);
+                        continue;
+                    }
+                    const Task &tk = s.tasks[k];
+                    char z = 0;
+                    if (!taboo[tk.i].compare_exchange_strong(z, 1)) continue;
+                    if (tk.type == 1) {
+                        z = 0;
+                        if (!taboo[tk.j].compare_exchange_strong(z, 1)) {
+                            taboo[tk.i].store(0);
+                            continue;
+                        }
+                    }
+                    char nd = 0;
+                    if (!done[k].compare_exchange_strong(nd, 1)) {
+                        taboo[tk.i].store(0);
+                        if (tk.type == 1) taboo[tk.j].store(0);
+                        continue;
+                    }
+                    if (tk.type == 0)
+                        do_self(s, s.cells[tk.i], vir[t]);
+                    else
+                        do_pair(s, tk.i, tk.j, vir[t]);
+                    taboo[tk.i].store(0);
+                    if (tk.type == 1) taboo[tk.j].store(0);
+                    remaining--;
+                    got = true;
+                    break;
+                }
+                if (!got) std::this_thread::yield();
+            }
+        });
+    for (auto &t : th) t.join();
+    for (double v : vir) vir_total += v;
+    s.virial = vir_total;
+}
+
+// bond_eval (tfBond.cpp:97-345), no decay or dissociation.
+void bonds_eval(System &s, double &epot_out) {
+    double epot = 0.0;
+    const int nb = (int)s.bonds.size() / 3;
+    for (int b = 0; b < nb; b++) {
+        Part &pi = s.part(s.bonds[3 * b]);
+        Part &pj = s.part(s.bonds[3 * b + 1]);
+        const Pot &pot = s.pots[s.bonds[3 * b + 2]];
+        const int *li = s.cells[s.where[pi.id].first].loc, *lj = s.cells[s.where[pj.id].first].loc;
+        double pix[3], dx[3];
+        for (int k = 0; k < 3; k++) {
+            int sh = li[k] - lj[k];
+            if (sh > 1)
+                sh = -1;
+            else if (sh < -1)
+                sh = 1;
+            pix[k] = pi.x[k] + s.h[k] * sh;
+            dx[k] = pix[k] - pj.x[k];
+        }
+        const double r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+        // potential_eval_adjust_distance2 (tf_potential_eval.h:52-55)
+        double r = std::sqrt(r2);
+        if (pot.flags & POT_SCALED)
+            r = r / (pi.radius + pj.radius);
+        else if (pot.flags & POT_SHIFTED)
+            r = r - (pi.radius + pj.radius) + pot.r0p1;
+        double rr2 = r * r;
+        if (rr2 > pot.b * pot.b) continue;
+        rr2 = std::fmax(rr2, pot.a * pot.a);
+        double e, eff;
+        eval_plain(pot, rr2, e, eff);
+        for (int k = 0; k < 3; k++) {
+            const double w = eff * dx[k];
+            pi.f[k] -= w;
+            pj.f[k] += w;
+        }
+        epot += e;
+    }
+    epot_out += epot;
+}
+
+// angle_eval (tfAngle.cpp:90-384).
+void angles_eval(System &s, double &epot_out) {
+    double epot = 0.0;
+    const int na = (int)s.angles.size() / 4;
+    for (int a = 0; a < na; a++) {
+        Part &pi = s.part(s.angles[4 * a]);
+        Part &pj = s.part(s.angles[4 * a + 1]);
+        Part &pk = s.part(s.angles[4 * a + 2]);
+        const Pot &pot = s.pots[s.angles[4 * a + 3]];
+        const int *li = s.cells[s.where[pi.id].first].loc, *lj = s.cells[s.where[pj.id].first].loc,
+                  *lk = s.cells[s.where[pk.id].first].loc;
+        double xi[3], xj[3], xk[3], rji[3], rjk[3];
+        for (int k = 0; k < 3; k++) {
+            xj[k] = pj.x[k];
+            int sh = li[k] - lj[k];
+            if (sh > 1)
+                sh = -1;
+            else if (sh < -1)
+                sh = 1;
+            xi[k] = pi.x[k] + sh * s.h[k];
+            sh = lk[k] - lj[k];
+            if (sh > 1)
+                sh = -1;
+            else if (sh < -1)
+                sh = 1;
+            xk[k] = pk.x[k] + sh * s.h[k];
+        }
+        for (int k = 0; k < 3; k++) {
+            rji[k] = xi[k] - xj[k];
+            rjk[k] = xk[k] - xj[k];
+        }
+        const double dprod = rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2];
+        const double inji = 1.0 / std::sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+        const double injk = 1.0 / std::sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+        const double ct = std::fmax(-1.0, std::fmin(1.0, dprod * inji * injk));
+        double dxi[3], dxk[3];
+        for (int k = 0; k < 3; k++) {
+            dxi[k] = (rjk[k] * injk - ct * rji[k] * inji) * inji;
+            dxk[k] = (rji[k] * inji - ct * rjk[k] * injk) * injk;
+        }
+        if (ct > pot.b) continue;
+        double e, eff;
+        if (!eval_r(pot, std::fmax(ct, pot.a), e, eff)) continue;
+        for (int k = 0; k < 3; k++) {
+            const double wi = eff * dxi[k], wk = eff * dxk[k];
+            pi.f[k] -= wi;
+            pk.f[k] -= wk;
+            pj.f[k] += wi + wk;
+        }
+        epot += e;
+    }
+    epot_out += epot;
+}
+
+// cell_advance_forward_euler (tf_engine_advance.cpp:140-247) + Fluxes_integrate.
+void advance_cell(System &s, int cid) {
+    Cell &c = s.cells[cid];
+    double maxx[3], maxx2[3], maxv[3], maxv2[3];
+    for (int k = 0; k < 3; k++) {
+        maxx[k] = s.h[k] * s.max_frac;
+        maxx2[k] = maxx[k] * maxx[k];
+        maxv[k] = maxx[k] / s.dt;
+        maxv2[k] = maxv[k] * maxv[k];
+    }
+    const double dt = s.dt;
+    size_t pid = 0;
+    while (pid < c.parts.size()) {
+        Part &p = c.parts[pid];
+        if ((p.flags & F_CLUSTER) ||
+            ((p.flags & F_FROZEN_X) && (p.flags & F_FROZEN_Y) && (p.flags & F_FROZEN_Z))) {
+            pid++;
+            continue;
+        }
+        const double mask[3] = {(p.flags & F_FROZEN_X) ? 0.0 : 1.0, (p.flags & F_FROZEN_Y) ? 0.0 : 1.0,
+                                (p.flags & F_FROZEN_Z) ? 0.0 : 1.0};
+        int delta[3];
+        if (s.types[p.type].dynamics == 0) {
+            for (int k = 0; k < 3; k++) {
+                const double v = mask[k] * (p.v[k] + dt * p.f[k] * p.imass);
+                p.v[k] = v * v <= maxv2[k] ? v : v / std::fabs(v) * maxv[k];
+                p.x[k] += dt * p.v[k];
+                delta[k] = std::isgreaterequal(p.x[k], s.h[k]) - std::isless(p.x[k], 0.0);
+            }
+        } else {
+            for (int k = 0; k < 3; k++) {
+                double dx = mask[k] * (dt * p.f[k] * p.imass);
+                dx = dx * dx <= maxx2[k] ? dx : dx / std::fabs(dx) * maxx[k];
+                p.v[k] = dx / dt;
+                p.x[k] += dx;
+                delta[k] = std::isgreaterequal(p.x[k], s.h[k]) - std::isless(p.x[k], 0.0);
+            }
+        }
+        if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
+            for (int k = 0; k < 3; k++) {
+                if (p.x[k] >= 2.0 * s.h[k] || p.x[k] <= -s.h[k]) s.toofast = 1;
+                const double dxk = -delta[k] * s.h[k];
+                p.x[k] += dxk;
+            }
+            Cell &dest = s.cells[s.cellid((c.loc[0] + delta[0] + s.cdim[0]) % s.cdim[0],
+                                          (c.loc[1] + delta[1] + s.cdim[1]) % s.cdim[1],
+                                          (c.loc[2] + delta[2] + s.cdim[2]) % s.cdim[2])];
+            {
+                std::lock_guard<std::mutex> g(dest.mtx);
+                dest.incoming.push_back(p);
+            }
+            if (pid + 1 < c.parts.size()) c.parts[pid] = c.parts.back();
+            c.parts.pop_back();
+        } else {
+            pid++;
+        }
+    }
+    // Fluxes_integrate over the particles left in this cell.
+    for (Part &p : c.parts)
+        for (int k = 0; k < s.nspecies; k++) {
+            p.s[k] = std::fmax(0.0, p.s[k] + s.dt_flux * p.q[k] * 1.0);
+            p.q[k] = 0;
+        }
+}
+
+double kinetic(const System &s) {
+    double total = 0.0;
+    for (const Cell &c : s.cells)
+        for (const Part &p : c.parts)
+            total += 0.5 * s.types[p.type].mass * (p.v[0] * p.v[0] + p.v[1] * p.v[1] + p.v[2] * p.v[2]);
+    return total;
+}
+
+void force(System &s, int nthreads) {
+    s.ekin = kinetic(s);
+    nonbond(s, nthreads);
+    s.reindex();
+    s.epot_bond = 0;
+    s.epot_angle = 0;
+    bonds_eval(s, s.epot_bond);
+    angles_eval(s, s.epot_angle);
+    s.epot_nb = 0;
+    for (const Cell &c : s.cells) s.epot_nb += c.epot;
+    s.epot = s.epot_nb + s.epot_bond + s.epot_angle;
+}
+
+void step(System &s, int nthreads) {
+    s.time += 1;
+    force(s, nthreads);
+    // staggered order (tf_engine_advance.cpp:122-136)
+    std::vector<int> ids;
+    ids.reserve(s.ncells);
+    for (int ii = 0; ii < 3; ii++)
+        for (int jj = 0; jj < 3; jj++)
+            for (int kk = 0; kk < 3; kk++)
+                for (int i = ii; i < s.cdim[0]; i += 3)
+                    for (int j = jj; j < s.cdim[1]; j += 3)
+                        for (int k = kk; k < s.cdim[2]; k += 3) ids.push_back(s.cellid(i, j, k));
+    if (nthreads <= 1) {
+        for (int c : ids) advance_cell(s, c);
+    } else {
+        std::atomic<int> next{0};
+        std::vector<std::thread> th;
+        for (int t = 0; t < nthreads; t++)
+            th.emplace_back([&]() {
+                for (int q = next++; q < (int)ids.size(); q = next++) advance_cell(s, ids[q]);
+            });
+        for (auto &t : th) t.join();
+    }
+    // space_cell_welcome
+    for (Cell &c : s.cells) {
+        for (Part &p : c.incoming) c.parts.push_back(p);
+        c.incoming.clear();
+    }
+    s.reindex();
+}
+
+}  // namespace
+
+extern "C" {
+
+void *or_create(const double origin[3], const double dim[3], const int cells[3], double cutoff, int max_types,
+                double dt, int nr_fluxsteps, double max_dist_fraction, int nspecies) {
+    if (nspecies < 0 || nspecies > kMaxS || max_types < 1 || nr_fluxsteps < 1) return nullptr;
+    System *s = new System();
+    if (build_space(*s, origin, dim, cells, cutoff) != 0) {
+        delete s;
+        return nullptr;
+    }
+    s->dt = dt;
+    s->dt_flux = dt / nr_fluxsteps;
+    s->max_frac = max_dist_fraction;
+    s->max_types = max_types;
+    s->potmap.assign((size_t)max_types * max_types, -1);
+    s->fluxmap.assign((size_t)max_types * max_types, {});
+    s->nspecies = nspecies;
+    return s;
+}
+
+void or_destroy(void *h) { delete static_cast<System *>(h); }
+
+int or_geometry(void *h, int cdim[3], double hh[3], int span[3]) {
+    System *s = static_cast<System *>(h);
+    for (int k = 0; k < 3; k++) {
+        cdim[k] = s->cdim[k];
+        hh[k] = s->h[k];
+        span[k] = s->span[k];
+    }
+    return (int)s->tasks.size();
+}
+
+int or_add_type(void *h, double mass, double radius, int dynamics) {
+    System *s = static_cast<System *>(h);
+    if ((int)s->types.size() >= s->max_types) return -1;
+    s->types.push_back({mass, 1.0 / mass, radius, dynamics});
+    return (int)s->types.size() - 1;
+}
+
+int or_add_table(void *h, unsigned flags, double a, double b, double r0p1, const double alpha[3], int n,
+                 const double *c) {
+    System *s = static_cast<System *>(h);
+    Pot p;
+    p.kind = KIND_TABLE;
+    p.flags = flags;
+    p.a = a;
+    p.b = b;
+    p.r0p1 = r0p1;
+    for (int k = 0; k < 3; k++) p.alpha[k] = alpha[k];
+    p.n = n;
+    p.c.assign(c, c + 8 * (size_t)(n + 1));
+    s->pots.push_back(p);
+    return (int)s->pots.size() - 1;
+}
+
+int or_add_dpd(void *h, double alpha, double gamma, double sigma, double a, double b, unsigned flags) {
+    System *s = static_cast<System *>(h);
+    Pot p;
+    p.kind = KIND_DPD;
+    p.flags = flags;
+    p.a = a;
+    p.b = b;
+    p.dalpha = alpha;
+    p.dgamma = gamma;
+    p.dsigma = sigma;
+    s->pots.push_back(p);
+    return (int)s->pots.size() - 1;
+}
+
+int or_bind(void *h, int pot, int ti, int tj) {
+    System *s = static_cast<System *>(h);
+    if (pot < 0 || pot >= (int)s->pots.size() || ti < 0 || tj < 0 || ti >= (int)s->types.size() ||
+        tj >= (int)s->types.size())
+        return -1;
+    s->potmap[ti * s->max_types + tj] = pot;
+    s->potmap[tj * s->max_types + ti] = pot;
+    return 0;
+}
+
+int or_add_flux(void *h, int kind, int ta, int tb, int ia, int ib, double k, double decay, double target,
+                double cutoff) {
+    System *s = static_cast<System *>(h);
+    Flux f{kind, ta, tb, ia, ib, k, decay, target, cutoff <= 0 ? s->cutoff : cutoff};
+    s->fluxes.push_back(f);
+    const int id = (int)s->fluxes.size() - 1;
+    s->fluxmap[ta * s->max_types + tb].push_back(id);
+    if (ta != tb) s->fluxmap[tb * s->max_types + ta].push_back(id);
+    return 0;
+}
+
+// space_setpartp (tfSpace.cpp:432-475) for each particle, then space_cell_add.
+int or_add_particles(void *h, int n, const int *types, const double *x, const double *v, const double *species) {
+    System *s = static_cast<System *>(h);
+    for (int q = 0; q < n; q++) {
+        Part p;
+        p.id = s->nparts + q;
+        p.type = types[q];
+        p.imass = s->types[p.type].imass;
+        p.radius = s->types[p.type].radius;
+        int ind[3];
+        double xx[3];
+        for (int k = 0; k < 3; k++) {
+            xx[k] = std::max<double>(s->origin[k] * (1.0 + DBL_EPSILON),
+                                     std::min<double>((s->dim[k] + s->origin[k]) * (1.0 - DBL_EPSILON), x[3 * q + k]));
+            ind[k] = int((xx[k] - s->origin[k]) * s->ih[k]);
+            if (ind[k] < 0 || ind[k] >= s->cdim[k]) return -1;
+        }
+        Cell &c = s->cells[s->cellid(ind[0], ind[1], ind[2])];
+        for (int k = 0; k < 3; k++) {
+            p.x[k] = xx[k] - c.origin[k];
+            p.v[k] = v ? v[3 * q + k] : 0.0;
+        }
+        if (species)
+            for (int k = 0; k < s->nspecies; k++) p.s[k] = species[(size_t)q * s->nspecies + k];
+        c.parts.push_back(p);
+    }
+    s->nparts += n;
+    s->reindex();
+    return 0;
+}
+
+int or_set_flags(void *h, int n, const int *ids, const unsigned *flags) {
+    System *s = static_cast<System *>(h);
+    for (int q = 0; q < n; q++) s->part(ids[q]).flags = flags[q];
+    return 0;
+}
+
+int or_add_bonds(void *h, int pot, int n, const int *ij) {
+    System *s = static_cast<System *>(h);
+    for (int b = 0; b < n; b++) {
+        s->bonds.push_back(ij[2 * b]);
+        s->bonds.push_back(ij[2 * b + 1]);
+        s->bonds.push_back(pot);
+    }
+    return 0;
+}
+
+int or_add_angles(void *h, int pot, int n, const int *ijk) {
+    System *s = static_cast<System *>(h);
+    for (int a = 0; a < n; a++) {
+        for (int k = 0; k < 3; k++) s->angles.push_back(ijk[3 * a + k]);
+        s->angles.push_back(pot);
+    }
+    return 0;
+}
+
+int or_set_dpd_noise(void *h, int mode, unsigned long long seed) {
+    System *s = static_cast<System *>(h);
+    s->dpd_noise = mode;
+    s->seed = seed;
+    return 0;
+}
+
+int or_step(void *h, int nsteps, int nthreads) {
+    System *s = static_cast<System *>(h);
+    for (int k = 0; k < nsteps; k++) step(*s, nthreads);
+    const int tf = s->toofast.exchange(0);
+    return tf ? -2 : 0;
+}
+
+// Force evaluation of the current state: kinetic energy, nonbonded and
+// bonded passes, without the update (time is not advanced).
+int or_force(void *h, int nthreads) {
+    System *s = static_cast<System *>(h);
+    s->time += 1;
+    force(*s, nthreads);
+    s->time -= 1;
+    return 0;
+}
+
+int or_read(void *h, int *cid, double *x, double *v, double *f, double *species, double *q) {
+    System *s = static_cast<System *>(h);
+    for (int id = 0; id < s->nparts; id++) {
+        const int c = s->where[id].first;
+        const Part &p = s->part(id);
+        if (cid) cid[id] = c;
+        for (int k = 0; k < 3; k++) {
+            if (x) x[3 * id + k] = p.x[k];
+            if (v) v[3 * id + k] = p.v[k];
+        }
+        if (f)
+            for (int k = 0; k < 4; k++) f[4 * id + k] = p.f[k];
+        for (int k = 0; k < s->nspecies; k++) {
+            if (species) species[(size_t)id * s->nspecies + k] = p.s[k];
+            if (q) q[(size_t)id * s->nspecies + k] = p.q[k];
+        }
+    }
+    return s->nparts;
+}
+
+int or_energies(void *h, double out[6]) {
+    System *s = static_cast<System *>(h);
+    out[0] = s->epot;
+    out[1] = s->epot_nb;
+    out[2] = s->epot_bond;
+    out[3] = s->epot_angle;
+    out[4] = s->ekin;
+    out[5] = s->virial;
+    return 0;
+}
+
+long long or_time(void *h) { return static_cast<System *>(h)->time; }
+
+int or_set_sweep_unsigned(void *h, int on) {
+    static_cast<System *>(h)->sweep_unsigned = on;
+    return 0;
+}
+
+void or_philox(unsigned c[4], unsigned k0, unsigned k1) { philox(c, k0, k1); }
+
+double or_dpd_xi(unsigned a, unsigned b, unsigned long long step, unsigned long long seed) {
+    return xi_of(a, b, step, seed);
+}
+
+// potential_eval_ex on a registered table, for closed-form tests.
+int or_eval_ex(void *h, int pot, double ri, double rj, double r2, double *e, double *f) {
+    System *s = static_cast<System *>(h);
+    return eval_ex(s->pots[pot], ri, rj, r2, *e, *f) ? 1 : 0;
+}
+
+int or_eval_r(void *h, int pot, double r, double *e, double *f) {
+    System *s = static_cast<System *>(h);
+    return eval_r(s->pots[pot], r, *e, *f) ? 1 : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,188 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+Error measures for per-particle forces F (3-vectors), over all particles:
+  scale   = rms |F_oracle|
+  err_rms = max_i |F_gpu,i - F_oracle,i| / scale
+  err_rel = |F_gpu,i - F_oracle,i| / max(|F_oracle,i|, 1e-3 * scale)  (max and p99)
+Tolerances (written per test):
+  FP64 path:  err_rms <= 1e-10, relative energy error <= 1e-11
+  mixed path: err_rms <= 1e-5,  p99(err_rel) <= 1e-5, relative energy error <= 1e-6
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def force_errors(fg, fo):
+    d = np.linalg.norm(fg[:, :3] - fo[:, :3], axis=1)
+    mag = np.linalg.norm(fo[:, :3], axis=1)
+    scale = math.sqrt(np.mean(mag ** 2))
+    rel = d / np.maximum(mag, 1e-3 * scale)
+    return dict(err_rms=float(d.max() / scale), rel_max=float(rel.max()), rel_p99=float(np.percentile(rel, 99)),
+                scale=scale)
+
+
+def run_force(tf, W, O, scene, precision, threads=4):
+    eng, pots = W.build_engine(scene)
+    orc = O.Oracle(scene, pots)
+    orc.force(threads)
+    eng.to_device(precision)
+    eng.force()
+    pg = eng.particles()
+    po = orc.particles()
+    eg = eng.energies()
+    eo = orc.energies()
+    eng.from_device()
+    return pg, po, eg, eo
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("precision,tol_f,tol_e", [(1, 1e-10, 1e-11), (0, 1e-5, 1e-6)])
+def test_force_cell_sorting(tf, W, O, gpu, precision, tol_f, tol_e):
+    scene = W.cell_sorting()
+    pg, po, eg, eo = run_force(tf, W, O, scene, precision)
+    assert np.array_equal(pg["cid"], po["cid"])
+    er = force_errors(pg["f"], po["f"])
+    print("cell_sorting prec", precision, er, eg["epot"], eo["epot"])
+    assert er["err_rms"] <= tol_f
+    assert rel(eg["epot"], eo["epot"]) <= tol_e
+    # number density (f[3]) against the oracle's exact-sqrt W sum
+    nd_err = np.abs(pg["f"][:, 3] - po["f"][:, 3]) / po["f"][:, 3]
+    assert nd_err.max() <= (1e-12 if precision == 1 else 1e-5)
+    assert rel(eg["virial"], eo["virial"]) <= tol_e * 10
+
+
+@pytest.mark.parametrize("precision,tol_f,tol_e", [(1, 1e-10, 1e-11), (0, 1e-5, 1e-6)])
+def test_force_morse_lj(tf, W, O, gpu, precision, tol_f, tol_e):
+    scene = W.morse_lj(n=40_000)
+    pg, po, eg, eo = run_force(tf, W, O, scene, precision)
+    er = force_errors(pg["f"], po["f"])
+    print("morse_lj prec", precision, er, eg["epot"], eo["epot"])
+    assert er["err_rms"] <= tol_f
+    if precision == 0:
+        assert er["rel_p99"] <= tol_f
+    assert rel(eg["epot"], eo["epot"]) <= tol_e
+
+
+@pytest.mark.parametrize("noise", [None, "philox"])
+@pytest.mark.parametrize("precision,tol_f", [(1, 1e-10), (0, 1e-5)])
+def test_force_dpd_flux(tf, W, O, gpu, noise, precision, tol_f):
+    scene = W.dpd_droplets(n=24_000, noise=noise)
+    pg, po, eg, eo = run_force(tf, W, O, scene, precision)
+    er = force_errors(pg["f"], po["f"])
+    print("dpd prec", precision, noise, er)
+    assert er["err_rms"] <= tol_f
+    # pending species flux after one evaluation
+    dq = np.abs(pg["species_q"] - po["species_q"]).max()
+    qs = np.abs(po["species_q"]).max()
+    assert dq <= tol_f * max(qs, 1e-12) * 10
+
+
+@pytest.mark.parametrize("precision,tol_f,tol_e", [(1, 1e-10, 1e-11), (0, 1e-5, 1e-6)])
+def test_force_bonded(tf, W, O, gpu, precision, tol_f, tol_e):
+    scene = W.bonded_chains(nchains=256, beads=32)
+    pg, po, eg, eo = run_force(tf, W, O, scene, precision)
+    er = force_errors(pg["f"], po["f"])
+    print("bonded prec", precision, er, eg, eo)
+    assert er["err_rms"] <= tol_f
+    for k in ("epot_nonbond", "epot_bond", "epot_angle"):
+        assert rel(eg[k], eo[k]) <= tol_e, k
+
+
+@pytest.mark.parametrize("scene_name", ["cell_sorting", "dpd", "bonded"])
+def test_trajectory_fp64(tf, W, O, gpu, scene_name):
+    """Ten whole steps on the FP64 path track the oracle (before divergence)."""
+    if scene_name == "cell_sorting":
+        scene = W.cell_sorting()
+    elif scene_name == "dpd":
+        scene = W.dpd_droplets(n=24_000, noise="philox")
+    else:
+        scene = W.bonded_chains(nchains=128, beads=32)
+    eng, pots = W.build_engine(scene)
+    orc = O.Oracle(scene, pots)
+    eng.to_device(1)
+    eng.step(10)
+    assert orc.step(10, 4) == 0
+    pg = eng.particles()
+    po = orc.particles()
+    xg = eng.positions()
+    eng.from_device()
+    assert np.array_equal(pg["cid"], po["cid"])
+    assert np.abs(pg["x"] - po["x"]).max() <= 1e-9
+    assert np.abs(pg["v"] - po["v"]).max() <= 1e-7 * max(1.0, np.abs(po["v"]).max())
+    assert xg.shape == (scene["x"].shape[0], 3)
+    eg, eo = eng.energies(), orc.energies()
+    assert rel(eg["epot"], eo["epot"]) <= 1e-9
+    assert rel(eg["ekin"], eo["ekin"]) <= 1e-9 or abs(eo["ekin"]) < 1e-12
+    if scene["nspecies"]:
+        assert np.abs(pg["species"] - po["species"]).max() <= 1e-9
+
+
+def test_trajectory_mixed_short(tf, W, O, gpu):
+    """Mixed precision stays close to the FP64 oracle over a few steps."""
+    scene = W.cell_sorting()
+    eng, pots = W.build_engine(scene)
+    orc = O.Oracle(scene, pots)
+    eng.to_device(0)
+    eng.step(3)
+    orc.step(3, 4)
+    pg, po = eng.particles(), orc.particles()
+    eng.from_device()
+    same = np.mean(pg["cid"] == po["cid"])
+    assert same >= 0.999
+    m = pg["cid"] == po["cid"]
+    assert np.abs(pg["x"][m] - po["x"][m]).max() <= 1e-4
+
+
+def test_determinism_and_ensemble(tf, W, O, gpu):
+    """Same inputs give bitwise-identical results; a system in an ensemble
+    gives the same results as alone; graph replay equals eager stepping."""
+    scenes = W.ensemble(nrep=6, nmax=3000)
+    outs = []
+    for sc in scenes:
+        eng, _ = W.build_engine(sc)
+        eng.to_device(0)
+        eng.step(5)
+        outs.append(eng.particles())
+        eng.from_device()
+    # again, alone: bitwise identical
+    eng, _ = W.build_engine(scenes[0])
+    eng.to_device(0)
+    eng.step(5)
+    again = eng.particles()
+    eng.from_device()
+    assert np.array_equal(again["x"], outs[0]["x"]) and np.array_equal(again["f"], outs[0]["f"])
+    for use_graph in (False, True):
+        engines = [W.build_engine(sc)[0] for sc in scenes]
+        ens = tf.Ensemble(engines, precision=0)
+        ens.step(5, use_graph=use_graph)
+        ens.close(download=True)
+        for k, e in enumerate(engines):
+            p = e.particles()
+            assert np.array_equal(p["cid"], outs[k]["cid"])
+            assert np.array_equal(p["x"], outs[k]["x"]), (use_graph, k)
+            assert np.array_equal(p["f"], outs[k]["f"]), (use_graph, k)
+
+
+def test_toggle_round_trip(tf, W, gpu):
+    """to_device / from_device keeps the state exactly (engine_toCUDA role)."""
+    scene = W.dpd_droplets(n=6000)
+    eng, _ = W.build_engine(scene)
+    before = eng.particles()
+    eng.to_device(1)
+    assert eng.on_device
+    eng.to_device(1)   # second call is a no-op
+    eng.from_device()
+    after = eng.particles()
+    assert np.array_equal(before["cid"], after["cid"])
+    assert np.array_equal(before["x"], after["x"])
+    assert np.array_equal(before["v"], after["v"])
+    assert np.array_equal(before["species"], after["species"])
+    with pytest.raises(tf.TfgError):
+        eng.step(1)   # not on the device any more

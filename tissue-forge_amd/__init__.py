@@ -1,0 +1,392 @@
+"""Python surface of the MI355X engine_step module (ctypes over libtfg.so).
+
+The object model follows the reference's Python API for this path
+(tf.init / tf.ParticleType / tf.Potential.* / tf.bind.types / tf.step /
+tf.Simulator.cuda_config.engine.to_device, docs/main/source/cuda.rst:56-89),
+but engines are explicit objects so that several systems can be batched.
+
+Stepping happens only on the GPU: Engine.step() raises when the engine has
+not been moved to the device, and loading fails loudly when libtfg.so is
+missing.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtfg.so")
+
+NEWTONIAN = 0
+OVERDAMPED = 1
+FLUX_FICK = 0
+FLUX_SECRETE = 1
+FLUX_UPTAKE = 2
+DPD_NOISE_UNIT = 0
+DPD_NOISE_PHILOX = 1
+PREC_MIXED = 0
+PREC_FP64 = 1
+
+FROZEN_X = 1 << 3
+FROZEN_Y = 1 << 4
+FROZEN_Z = 1 << 5
+
+_lib = None
+
+
+class TfgError(RuntimeError):
+    """Raised when a C-ABI call returns a failure code."""
+
+    def __init__(self, code, msg):
+        super().__init__("tfg error 0x%08x: %s" % (code & 0xFFFFFFFF, msg))
+        self.code = code
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+
+_SIGNATURES = {
+    "tfg_last_error": (ctypes.c_char_p, []),
+    "tfg_version": (ctypes.c_char_p, []),
+    "tfg_engine_create": (ctypes.c_int, [ctypes.POINTER(_vp), _dp, _dp, _ip, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_int]),
+    "tfg_engine_destroy": (ctypes.c_int, [_vp]),
+    "tfg_engine_geometry": (ctypes.c_int, [_vp, _ip, _dp, _ip]),
+    "tfg_engine_set_max_distance_fraction": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "tfg_engine_set_nspecies": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfg_engine_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64]),
+    "tfg_engine_time": (ctypes.c_int64, [_vp]),
+    "tfg_engine_addtype": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "tfg_potential_morse": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 6 + [ctypes.c_int]),
+    "tfg_potential_lennard_jones_12_6": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 5),
+    "tfg_potential_harmonic": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 5),
+    "tfg_potential_harmonic_angle": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 5),
+    "tfg_potential_dpd": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 4 + [ctypes.c_int]),
+    "tfg_potential_destroy": (ctypes.c_int, [_vp]),
+    "tfg_potential_info": (ctypes.c_int, [_vp, _ip, _u32p, _ip, _dp, _dp, _dp, _dp]),
+    "tfg_potential_coeffs": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "tfg_potential_eval": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]),
+    "tfg_engine_addpot": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "tfg_engine_addflux": (ctypes.c_int, [_vp] + [ctypes.c_int] * 5 + [ctypes.c_double] * 4),
+    "tfg_engine_addparts": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _i32p]),
+    "tfg_engine_set_particle_flags": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _u32p]),
+    "tfg_engine_nparts": (ctypes.c_int, [_vp]),
+    "tfg_engine_addbonds": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _i32p]),
+    "tfg_engine_addangles": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _i32p]),
+    "tfg_device_count": (ctypes.c_int, []),
+    "tfg_device_set": (ctypes.c_int, [ctypes.c_int]),
+    "tfg_engine_to_device": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfg_engine_from_device": (ctypes.c_int, [_vp]),
+    "tfg_engine_on_device": (ctypes.c_int, [_vp]),
+    "tfg_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "tfg_engine_step": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfg_nonbond_eval": (ctypes.c_int, [_vp]),
+    "tfg_read_energies": (ctypes.c_int, [_vp, _dp]),
+    "tfg_read_particles": (ctypes.c_int, [_vp, _i32p, _dp, _dp, _dp, _dp, _dp]),
+    "tfg_read_positions": (ctypes.c_int, [_vp, _dp]),
+    "tfg_read_species_totals": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "tfg_last_step_ms": (ctypes.c_double, [_vp]),
+    "tfg_kernel_times": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "tfg_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfg_ensemble_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]),
+    "tfg_ensemble_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "tfg_ensemble_destroy": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfg_ensemble_last_ms": (ctypes.c_double, [_vp]),
+    "tfg_ensemble_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+}
+
+
+def lib():
+    """Loads libtfg.so (built in-tree by build.py); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libtfg.so not built: run `python tissue-forge_amd/build.py` (%s)" % LIB_PATH)
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def _check(rc):
+    if rc < 0:
+        raise TfgError(rc, lib().tfg_last_error().decode())
+    return rc
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, shape=None):
+    arr = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        arr = arr.reshape(shape)
+    return arr
+
+
+def device_count():
+    return lib().tfg_device_count()
+
+
+class Potential:
+    """A tabulated (or DPD) potential; constructors mirror tf.Potential.*."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.tfg_potential_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def _make(fn, *args):
+        h = _vp()
+        _check(fn(ctypes.byref(h), *args))
+        return Potential(h)
+
+    @staticmethod
+    def morse(d=1.0, a=6.0, r0=0.0, min=0.0001, max=3.0, tol=0.001, shifted=True):
+        return Potential._make(lib().tfg_potential_morse, d, a, r0, min, max, tol, 1 if shifted else 0)
+
+    @staticmethod
+    def lennard_jones_12_6(min, max, A, B, tol=None):
+        return Potential._make(lib().tfg_potential_lennard_jones_12_6, min, max, A, B,
+                               -1.0 if tol is None else tol)
+
+    @staticmethod
+    def harmonic(k, r0, min=None, max=None, tol=None):
+        return Potential._make(lib().tfg_potential_harmonic, k, r0, math.nan if min is None else min,
+                               math.nan if max is None else max, -1.0 if tol is None else tol)
+
+    @staticmethod
+    def harmonic_angle(k, theta0, min=None, max=None, tol=None):
+        return Potential._make(lib().tfg_potential_harmonic_angle, k, theta0, math.nan if min is None else min,
+                               math.nan if max is None else max, -1.0 if tol is None else tol)
+
+    @staticmethod
+    def dpd(alpha=1.0, gamma=1.0, sigma=1.0, cutoff=1.0, shifted=False):
+        return Potential._make(lib().tfg_potential_dpd, alpha, gamma, sigma, cutoff, 1 if shifted else 0)
+
+    def info(self):
+        kind = ctypes.c_int()
+        flags = ctypes.c_uint32()
+        n = ctypes.c_int()
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        r0p1 = ctypes.c_double()
+        alpha = (ctypes.c_double * 3)()
+        _check(lib().tfg_potential_info(self._h, ctypes.byref(kind), ctypes.byref(flags), ctypes.byref(n),
+                                        ctypes.byref(a), ctypes.byref(b), ctypes.byref(r0p1), alpha))
+        return dict(kind=kind.value, flags=flags.value, n=n.value, a=a.value, b=b.value,
+                    r0_plusone=r0p1.value, alpha=list(alpha))
+
+    def coeffs(self):
+        cnt = _check(lib().tfg_potential_coeffs(self._h, None, 0))
+        out = np.zeros(cnt, dtype=np.float64)
+        lib().tfg_potential_coeffs(self._h, _dptr(out), cnt)
+        return out
+
+    def __call__(self, r, ri=0.0, rj=0.0):
+        """Energy and force/r at distance r (potential_eval_ex semantics)."""
+        e = ctypes.c_double()
+        f = ctypes.c_double()
+        _check(lib().tfg_potential_eval(self._h, r, ri, rj, ctypes.byref(e), ctypes.byref(f)))
+        return e.value, f.value
+
+
+class Engine:
+    """One simulated system: engine_init + the Universe configuration."""
+
+    def __init__(self, dim, cells, cutoff, dt=0.01, max_types=8, nr_fluxsteps=1, origin=(0.0, 0.0, 0.0),
+                 nspecies=0, max_distance_fraction=None):
+        L = lib()
+        self._h = _vp()
+        o = _f64(origin, 3)
+        d = _f64(dim, 3)
+        c = np.ascontiguousarray(cells, dtype=np.int32).reshape(3)
+        _check(L.tfg_engine_create(ctypes.byref(self._h), _dptr(o), _dptr(d), c.ctypes.data_as(_ip), cutoff,
+                                   max_types, dt, nr_fluxsteps))
+        self._pots = []
+        self.nspecies = nspecies
+        if nspecies:
+            _check(L.tfg_engine_set_nspecies(self._h, nspecies))
+        if max_distance_fraction is not None:
+            _check(L.tfg_engine_set_max_distance_fraction(self._h, max_distance_fraction))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(lib().tfg_engine_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and _lib is not None:
+                _lib.tfg_engine_destroy(self._h)
+        except Exception:
+            pass
+
+    # ---- model ----
+    def geometry(self):
+        cd = (ctypes.c_int * 3)()
+        h = (ctypes.c_double * 3)()
+        sp = (ctypes.c_int * 3)()
+        _check(lib().tfg_engine_geometry(self._h, cd, h, sp))
+        return list(cd), list(h), list(sp)
+
+    def add_type(self, name, mass=1.0, radius=1.0, dynamics=NEWTONIAN):
+        return _check(lib().tfg_engine_addtype(self._h, name.encode(), mass, radius, dynamics))
+
+    def bind(self, pot, ti, tj):
+        self._pots.append(pot)
+        _check(lib().tfg_engine_addpot(self._h, pot._h, ti, tj))
+
+    def add_flux(self, kind, ta, tb, sa, sb, k, decay=0.0, target=0.0, cutoff=0.0):
+        _check(lib().tfg_engine_addflux(self._h, kind, ta, tb, sa, sb, k, decay, target, cutoff))
+
+    def set_dpd_noise(self, mode, seed=0):
+        _check(lib().tfg_engine_set_dpd_noise(self._h, mode, seed))
+
+    def add_particles(self, types, x, v=None, species=None):
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        n = types.shape[0]
+        x = _f64(x, (n, 3))
+        vp = None
+        if v is not None:
+            v = _f64(v, (n, 3))
+            vp = _dptr(v)
+        sp = None
+        if species is not None:
+            species = _f64(species, (n, self.nspecies))
+            sp = _dptr(species)
+        ids = np.zeros(n, dtype=np.int32)
+        _check(lib().tfg_engine_addparts(self._h, n, types.ctypes.data_as(_ip), _dptr(x), vp, sp,
+                                         ids.ctypes.data_as(_i32p)))
+        return ids
+
+    def set_flags(self, ids, flags):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        flags = np.ascontiguousarray(flags, dtype=np.uint32)
+        _check(lib().tfg_engine_set_particle_flags(self._h, ids.shape[0], ids.ctypes.data_as(_i32p),
+                                                   flags.ctypes.data_as(_u32p)))
+
+    def add_bonds(self, pot, ij):
+        self._pots.append(pot)
+        ij = np.ascontiguousarray(ij, dtype=np.int32).reshape(-1, 2)
+        _check(lib().tfg_engine_addbonds(self._h, pot._h, ij.shape[0], ij.ctypes.data_as(_i32p)))
+
+    def add_angles(self, pot, ijk):
+        self._pots.append(pot)
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        _check(lib().tfg_engine_addangles(self._h, pot._h, ijk.shape[0], ijk.ctypes.data_as(_i32p)))
+
+    @property
+    def nparts(self):
+        return _check(lib().tfg_engine_nparts(self._h))
+
+    @property
+    def time(self):
+        return lib().tfg_engine_time(self._h)
+
+    # ---- device toggle (cuda_config.engine.to_device / from_device) ----
+    def to_device(self, precision=PREC_MIXED):
+        if isinstance(precision, str):
+            precision = {"mixed": PREC_MIXED, "fp32": PREC_MIXED, "fp64": PREC_FP64}[precision]
+        _check(lib().tfg_engine_to_device(self._h, precision))
+
+    def from_device(self):
+        _check(lib().tfg_engine_from_device(self._h))
+
+    @property
+    def on_device(self):
+        return bool(lib().tfg_engine_on_device(self._h))
+
+    def set_launch(self, blocks=0, threads=0):
+        _check(lib().tfg_set_launch(self._h, blocks, threads))
+
+    # ---- stepping ----
+    def step(self, nsteps=1):
+        _check(lib().tfg_engine_step(self._h, nsteps))
+
+    def force(self):
+        _check(lib().tfg_nonbond_eval(self._h))
+
+    def last_step_ms(self):
+        return lib().tfg_last_step_ms(self._h)
+
+    def set_profiling(self, on=True):
+        _check(lib().tfg_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        out = np.zeros(8)
+        n = _check(lib().tfg_kernel_times(self._h, _dptr(out), 8))
+        names = ["bin", "nonbond", "advance", "bin_avg", "nonbond_avg", "advance_avg"]
+        return dict(zip(names, out[:n].tolist()))
+
+    # ---- readback ----
+    def energies(self):
+        out = np.zeros(6)
+        _check(lib().tfg_read_energies(self._h, _dptr(out)))
+        return dict(zip(["epot", "epot_nonbond", "epot_bond", "epot_angle", "ekin", "virial"], out.tolist()))
+
+    def particles(self):
+        n = self.nparts
+        cid = np.zeros(n, dtype=np.int32)
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        f = np.zeros((n, 4))
+        sp = np.zeros((n, max(self.nspecies, 1)))
+        q = np.zeros((n, max(self.nspecies, 1)))
+        _check(lib().tfg_read_particles(self._h, cid.ctypes.data_as(_i32p), _dptr(x), _dptr(v), _dptr(f),
+                                        _dptr(sp), _dptr(q)))
+        out = dict(cid=cid, x=x, v=v, f=f)
+        if self.nspecies:
+            out["species"] = sp[:, :self.nspecies]
+            out["species_q"] = q[:, :self.nspecies]
+        return out
+
+    def positions(self):
+        xg = np.zeros((self.nparts, 3))
+        _check(lib().tfg_read_positions(self._h, _dptr(xg)))
+        return xg
+
+    def species_totals(self):
+        out = np.zeros(8)
+        n = _check(lib().tfg_read_species_totals(self._h, _dptr(out), 8))
+        return out[:n]
+
+
+class Ensemble:
+    """Independent engines advanced together by the same device launches."""
+
+    def __init__(self, engines, precision=PREC_MIXED):
+        self.engines = list(engines)
+        arr = (_vp * len(self.engines))(*[e._h for e in self.engines])
+        self._h = _vp()
+        _check(lib().tfg_ensemble_create(ctypes.byref(self._h), arr, len(self.engines), precision))
+
+    def step(self, nsteps=1, use_graph=True):
+        _check(lib().tfg_ensemble_step(self._h, nsteps, 1 if use_graph else 0))
+
+    def last_ms(self):
+        return lib().tfg_ensemble_last_ms(self._h)
+
+    def set_launch(self, blocks=0, threads=0):
+        _check(lib().tfg_ensemble_set_launch(self._h, blocks, threads))
+
+    def close(self, download=True):
+        if self._h:
+            _check(lib().tfg_ensemble_destroy(self._h, 1 if download else 0))
+            self._h = None

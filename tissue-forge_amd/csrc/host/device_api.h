@@ -1,0 +1,45 @@
+// Interface between the host engine and the device module (device.hip).
+// Everything here is host-callable C++; the kernels stay private to the
+// device module.
+#pragma once
+
+#include <cstdint>
+
+namespace tfg {
+
+class Engine;
+struct Energies;
+struct DeviceEnsemble;
+
+enum Precision : int { PREC_MIXED = 0, PREC_FP64 = 1 };
+
+int device_select(int ordinal);
+int device_count();
+
+// Uploads the engines as independent systems of one ensemble.  Every engine
+// keeps a pointer to the ensemble until ensemble_destroy.
+int ensemble_create(DeviceEnsemble **out, Engine *const *engines, int n, int precision);
+int ensemble_destroy(DeviceEnsemble *ens, bool download);
+// Copies the device state of every system back into its host engine.
+int ensemble_download(DeviceEnsemble *ens);
+int ensemble_download_one(DeviceEnsemble *ens, int sys);
+// Whole steps: binning, nonbonded, bonded, flux, forward Euler.
+int ensemble_step(DeviceEnsemble *ens, int nsteps, bool use_graph);
+// Binning + nonbonded + bonded force evaluation only (no update).
+int ensemble_force(DeviceEnsemble *ens);
+int ensemble_energies(DeviceEnsemble *ens, int sys, Energies *out);
+// FP64 per-species totals of one system after the last step; returns count.
+int ensemble_species_totals(DeviceEnsemble *ens, int sys, double *out, int cap);
+// Re-uploads potentials / fluxes of every system (engine_addpot while on the device).
+int ensemble_refresh_tables(DeviceEnsemble *ens);
+int ensemble_set_launch(DeviceEnsemble *ens, int blocks, int threads);
+// Per-kernel timing of the last ensemble_step / ensemble_force call.
+int ensemble_timing(DeviceEnsemble *ens, double *ms_out, int cap, const char **names_out);
+// Milliseconds of device time for the last nsteps of ensemble_step.
+double ensemble_last_ms(DeviceEnsemble *ens);
+int ensemble_precision(DeviceEnsemble *ens);
+int ensemble_size(DeviceEnsemble *ens);
+// Per-step phase timing with HIP events on the module's stream (eager steps).
+int ensemble_set_profile(DeviceEnsemble *ens, bool on);
+
+}  // namespace tfg

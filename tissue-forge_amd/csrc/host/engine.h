@@ -1,0 +1,125 @@
+// Host engine: the model objects of one simulated system and the device
+// toggle.  Mirrors the parts of the reference's `struct engine` / `struct
+// space` / Universe that the engine_step hot path reads
+// (source/mdcore/include/tfEngine.h:164-418, tfSpace.h:127-243).
+//
+// The host engine owns every object.  Stepping is done only by the device
+// module: the reference's own CPU engine is not reimplemented here.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "potential.h"
+
+namespace tfg {
+
+struct DeviceEnsemble;   // device module (device.hip)
+
+// Particle flags (tfParticle.h:69-77).
+enum ParticleFlags : uint32_t {
+    PFLAG_GHOST = 1u << 0,
+    PFLAG_CLUSTER = 1u << 1,
+    PFLAG_BOUND = 1u << 2,
+    PFLAG_FROZEN_X = 1u << 3,
+    PFLAG_FROZEN_Y = 1u << 4,
+    PFLAG_FROZEN_Z = 1u << 5,
+};
+
+enum Dynamics : int { DYN_NEWTONIAN = 0, DYN_OVERDAMPED = 1 };   // tfParticle.h:63-64
+enum FluxKind : int { FLUX_FICK = 0, FLUX_SECRETE = 1, FLUX_UPTAKE = 2 };   // tfFlux.h:40-42
+enum DpdNoise : int { DPD_NOISE_UNIT = 0, DPD_NOISE_PHILOX = 1 };
+
+struct ParticleType {
+    std::string name;
+    double mass = 1.0, imass = 1.0, radius = 1.0;
+    int dynamics = DYN_NEWTONIAN;
+};
+
+struct FluxEntry {
+    int kind = FLUX_FICK;
+    int type_a = 0, type_b = 0;
+    int index_a = 0, index_b = 0;
+    double coef = 0, decay = 0, target = 0, cutoff = 0;
+};
+
+struct BondRec { int32_t i, j; int pot; };
+struct AngleRec { int32_t i, j, k; int pot; };
+
+// Cell-grid geometry, computed exactly as space_init / engine_init do
+// (tfEngine.cpp:1722-1727, tfSpace.cpp:737-770, 893-999).
+struct Geometry {
+    double origin[3] = {0, 0, 0};
+    double dim[3] = {10, 10, 10};
+    int cdim[3] = {1, 1, 1};
+    double h[3] = {10, 10, 10};
+    double ih[3] = {0.1, 0.1, 0.1};
+    int span[3] = {1, 1, 1};
+    double cutoff = 1.0, cutoff2 = 1.0;
+    int ncells = 1;
+    // Neighbour-cell offsets (dx, dy, dz) within span whose closest distance
+    // is inside the cutoff (the task-building cull of tfSpace.cpp:949-954).
+    std::vector<int> stencil;   // 3 ints per offset, includes (0, 0, 0)
+};
+
+int make_geometry(Geometry &g, const double origin[3], const double dim[3],
+                  const int cells[3], double cutoff);
+
+// Per-particle host record, indexed by particle id.
+struct HostParticle {
+    int32_t type = 0;
+    uint32_t flags = 0;
+    int32_t cid = 0;          // cell index (i*cdim1 + j)*cdim2 + k
+    double x[3] = {0, 0, 0};  // relative to the cell origin
+    double v[3] = {0, 0, 0};
+    double f[4] = {0, 0, 0, 0};   // force and number density of the last evaluation
+    double imass = 1.0, radius = 1.0;
+};
+
+struct Energies {
+    double epot = 0, epot_nonbond = 0, epot_bond = 0, epot_angle = 0, ekin = 0, virial = 0;
+};
+
+class Engine {
+  public:
+    Geometry geom;
+    std::vector<ParticleType> types;
+    int max_types = 64;
+    std::vector<std::shared_ptr<Potential>> pots;   // registry, index = pot id
+    std::vector<int> potmap;                        // max_types^2 -> pot id or -1
+    std::vector<FluxEntry> fluxes;
+    int nspecies = 0;
+    std::vector<HostParticle> parts;
+    std::vector<double> species;   // nspecies per particle
+    std::vector<double> species_q; // pending flux per particle
+    std::vector<BondRec> bonds;
+    std::vector<AngleRec> angles;
+
+    double dt = 0.01;
+    int nr_fluxsteps = 1;
+    double dt_flux = 0.01;
+    double max_dist_fraction = 0.05;   // tfEngine.cpp:1865
+    int64_t time = 0;                  // step counter (engine->time)
+    int dpd_noise = DPD_NOISE_UNIT;
+    uint64_t dpd_seed = 0;
+
+    Energies energies;
+    DeviceEnsemble *device = nullptr;   // non-null while on the device
+    int device_index = -1;              // position inside that ensemble
+
+    int init(const double origin[3], const double dim[3], const int cells[3],
+             double cutoff, int max_types, double dt, int nr_fluxsteps);
+    int add_type(const std::string &name, double mass, double radius, int dynamics);
+    int add_potential(const Potential &p);
+    int bind(int pot, int ti, int tj);
+    int add_flux(int kind, int ta, int tb, int ia, int ib, double k, double decay,
+                 double target, double cutoff);
+    int add_particles(int n, const int *types, const double *x, const double *v,
+                      const double *sp, int32_t *ids_out);
+    int add_bonds(int pot, int n, const int32_t *ij);
+    int add_angles(int pot, int n, const int32_t *ijk);
+};
+
+}  // namespace tfg

@@ -1,0 +1,173 @@
+"""Seeded synthetic scenes for the engine_step workloads.
+
+A scene is a plain dict (geometry, types, potential parameters, bindings,
+fluxes, particles, bonds, angles) so the same inputs can be fed to the device
+engine (build_engine) and to any independent checker.
+
+Workloads (particle counts are the full sizes; every builder takes a scale):
+  cell_sorting    8,000 particles, A/B 50/50, periodic cube of side 20,
+                  Morse d_AA = d_BB = 3, d_AB = 0.5, a = 5, r0 = 1, cutoff 3,
+                  overdamped, dt 0.005 (examples/py/cell_sorting.py of the
+                  reference: mass 40, radius 0.4).
+  dpd_droplets    250,000 particles at density 3, DPD alpha 25, gamma 4.5,
+                  sigma 3, cutoff 1; two droplets of radius 10 moving at +-1;
+                  one species with Fick flux 0.1 (0 in droplet 1, 1 in
+                  droplet 2, 0.5 in the solvent); dt 0.01.
+  bonded_chains   31,250 chains x 32 beads at density 0.8, LJ 12-6 (eps 1,
+                  sigma 1, cutoff 2.5), harmonic bonds k 100 r0 1, harmonic
+                  angles k 20 theta0 pi, dt 0.002.
+  morse_lj        2,000,000 particles of 4 types at density 1, Morse between
+                  like types, LJ between unlike types, cutoff 2.5.
+  ensemble        512 cell-sorting replicas, N log-uniform in [500, 20000],
+                  density 1, d_AB uniform in [0.2, 2.0].
+"""
+import math
+
+import numpy as np
+
+from . import (DPD_NOISE_PHILOX, FLUX_FICK, NEWTONIAN, OVERDAMPED, Engine, Potential)
+
+
+def _cells_for(L, rc):
+    return max(3, int(math.floor(L / rc)))
+
+
+def cell_sorting(n=8000, L=20.0, cells=4, seed=0, dt=0.005, d_ab=0.5, cutoff=3.0):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, L, size=(n, 3))
+    ptype = (np.arange(n) % 2).astype(np.int32)
+    return dict(
+        name="cell_sorting", dim=[L, L, L], cells=[cells] * 3, cutoff=cutoff, dt=dt, nspecies=0,
+        types=[dict(name="A", mass=40.0, radius=0.4, dynamics=OVERDAMPED),
+               dict(name="B", mass=40.0, radius=0.4, dynamics=OVERDAMPED)],
+        pots=[dict(kind="morse", d=3.0, a=5.0, r0=1.0, min=0.5, max=3.0, tol=1e-3, shifted=False),
+              dict(kind="morse", d=3.0, a=5.0, r0=1.0, min=0.5, max=3.0, tol=1e-3, shifted=False),
+              dict(kind="morse", d=d_ab, a=5.0, r0=1.0, min=0.5, max=3.0, tol=1e-3, shifted=False)],
+        bind=[(0, 0, 0), (1, 1, 1), (2, 0, 1)],
+        fluxes=[], x=x, v=np.zeros((n, 3)), ptype=ptype, species=None, bonds=None, angles=None,
+        dpd_noise=None)
+
+
+def morse_lj(n=2_000_000, density=1.0, seed=4, cells=None, dt=0.005, cutoff=2.5):
+    L = (n / density) ** (1.0 / 3.0)
+    if cells is None:
+        cells = _cells_for(L, cutoff)
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, L, size=(n, 3))
+    ptype = rng.integers(0, 4, size=n).astype(np.int32)
+    types = [dict(name="T%d" % t, mass=40.0, radius=0.4, dynamics=OVERDAMPED) for t in range(4)]
+    pots = [dict(kind="morse", d=3.0, a=5.0, r0=1.0, min=0.5, max=cutoff, tol=1e-3, shifted=False),
+            dict(kind="lj", min=0.8, max=cutoff, A=4.0, B=4.0, tol=None)]
+    bind = [(0, t, t) for t in range(4)] + [(1, a, b) for a in range(4) for b in range(a + 1, 4)]
+    return dict(name="morse_lj", dim=[L] * 3, cells=[cells] * 3, cutoff=cutoff, dt=dt, nspecies=0,
+                types=types, pots=pots, bind=bind, fluxes=[], x=x, v=np.zeros((n, 3)), ptype=ptype,
+                species=None, bonds=None, angles=None, dpd_noise=None)
+
+
+def dpd_droplets(n=250_000, density=3.0, seed=2, radius=10.0, dt=0.01, noise=None, noise_seed=1234):
+    L = (n / density) ** (1.0 / 3.0)
+    cutoff = 1.0
+    cells = _cells_for(L, cutoff)
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, L, size=(n, 3))
+    c1 = np.array([L / 4, L / 2, L / 2])
+    c2 = np.array([3 * L / 4, L / 2, L / 2])
+    r = min(radius, L / 4 - 1e-9)
+    in1 = np.sum((x - c1) ** 2, axis=1) < r * r
+    in2 = np.sum((x - c2) ** 2, axis=1) < r * r
+    ptype = np.zeros(n, dtype=np.int32)
+    ptype[in1] = 1
+    ptype[in2] = 2
+    v = np.zeros((n, 3))
+    v[in1, 0] = 1.0
+    v[in2, 0] = -1.0
+    species = np.full((n, 1), 0.5)
+    species[in1, 0] = 0.0
+    species[in2, 0] = 1.0
+    types = [dict(name=nm, mass=1.0, radius=0.5, dynamics=NEWTONIAN) for nm in ("solvent", "drop1", "drop2")]
+    pots = [dict(kind="dpd", alpha=25.0, gamma=4.5, sigma=3.0, cutoff=cutoff, shifted=False)]
+    bind = [(0, a, b) for a in range(3) for b in range(a, 3)]
+    fluxes = [dict(kind=FLUX_FICK, ta=a, tb=b, sa=0, sb=0, k=0.1, decay=0.0, target=0.0, cutoff=0.0)
+              for a in range(3) for b in range(a, 3)]
+    return dict(name="dpd_droplets", dim=[L] * 3, cells=[cells] * 3, cutoff=cutoff, dt=dt, nspecies=1,
+                types=types, pots=pots, bind=bind, fluxes=fluxes, x=x, v=v, ptype=ptype, species=species,
+                bonds=None, angles=None,
+                dpd_noise=None if noise is None else (DPD_NOISE_PHILOX if noise == "philox" else 0, noise_seed))
+
+
+def bonded_chains(nchains=31_250, beads=32, density=0.8, seed=3, dt=0.002, cutoff=2.5):
+    n = nchains * beads
+    L = (n / density) ** (1.0 / 3.0)
+    cells = _cells_for(L, cutoff)
+    rng = np.random.default_rng(seed)
+    start = rng.uniform(0.0, L, size=(nchains, 1, 3))
+    d = rng.normal(size=(nchains, beads - 1, 3))
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    steps = np.concatenate([np.zeros((nchains, 1, 3)), np.cumsum(d, axis=1)], axis=1)
+    x = np.mod(start + steps, L).reshape(n, 3)
+    ids = np.arange(n).reshape(nchains, beads)
+    ij = np.stack([ids[:, :-1].ravel(), ids[:, 1:].ravel()], axis=1)
+    ijk = np.stack([ids[:, :-2].ravel(), ids[:, 1:-1].ravel(), ids[:, 2:].ravel()], axis=1)
+    types = [dict(name="bead", mass=1.0, radius=0.5, dynamics=NEWTONIAN)]
+    pots = [dict(kind="lj", min=0.8, max=cutoff, A=4.0, B=4.0, tol=None),
+            dict(kind="harmonic", k=100.0, r0=1.0),
+            dict(kind="harmonic_angle", k=20.0, theta0=math.pi)]
+    return dict(name="bonded_chains", dim=[L] * 3, cells=[cells] * 3, cutoff=cutoff, dt=dt, nspecies=0,
+                types=types, pots=pots, bind=[(0, 0, 0)], fluxes=[], x=x, v=np.zeros((n, 3)),
+                ptype=np.zeros(n, dtype=np.int32), species=None, bonds=dict(pot=1, ij=ij),
+                angles=dict(pot=2, ijk=ijk), dpd_noise=None)
+
+
+def ensemble(nrep=512, seed=5, nmin=500, nmax=20000, dt=0.005):
+    rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=nrep))).astype(int)
+    dabs = rng.uniform(0.2, 2.0, size=nrep)
+    scenes = []
+    for r in range(nrep):
+        n = int(ns[r])
+        L = float(n) ** (1.0 / 3.0)
+        rc = min(3.0, L / 3.0)
+        cells = _cells_for(L, rc)
+        sc = cell_sorting(n=n, L=L, cells=cells, seed=seed * 100003 + r, dt=dt, d_ab=float(dabs[r]), cutoff=rc)
+        sc["name"] = "replica%d" % r
+        scenes.append(sc)
+    return scenes
+
+
+def make_potential(p):
+    kind = p["kind"]
+    if kind == "morse":
+        return Potential.morse(d=p["d"], a=p["a"], r0=p["r0"], min=p["min"], max=p["max"], tol=p["tol"],
+                               shifted=p["shifted"])
+    if kind == "lj":
+        return Potential.lennard_jones_12_6(p["min"], p["max"], p["A"], p["B"], tol=p.get("tol"))
+    if kind == "harmonic":
+        return Potential.harmonic(p["k"], p["r0"], min=p.get("min"), max=p.get("max"), tol=p.get("tol"))
+    if kind == "harmonic_angle":
+        return Potential.harmonic_angle(p["k"], p["theta0"], min=p.get("min"), max=p.get("max"), tol=p.get("tol"))
+    if kind == "dpd":
+        return Potential.dpd(alpha=p["alpha"], gamma=p["gamma"], sigma=p["sigma"], cutoff=p["cutoff"],
+                             shifted=p["shifted"])
+    raise ValueError("unknown potential kind %r" % kind)
+
+
+def build_engine(scene, potentials=None, max_distance_fraction=None):
+    """Creates a host Engine for the scene; returns (engine, potentials)."""
+    pots = potentials if potentials is not None else [make_potential(p) for p in scene["pots"]]
+    eng = Engine(dim=scene["dim"], cells=scene["cells"], cutoff=scene["cutoff"], dt=scene["dt"],
+                 max_types=max(8, len(scene["types"])), nspecies=scene["nspecies"],
+                 max_distance_fraction=max_distance_fraction)
+    for t in scene["types"]:
+        eng.add_type(t["name"], mass=t["mass"], radius=t["radius"], dynamics=t["dynamics"])
+    for pi, a, b in scene["bind"]:
+        eng.bind(pots[pi], a, b)
+    for f in scene["fluxes"]:
+        eng.add_flux(f["kind"], f["ta"], f["tb"], f["sa"], f["sb"], f["k"], f["decay"], f["target"], f["cutoff"])
+    if scene.get("dpd_noise"):
+        eng.set_dpd_noise(*scene["dpd_noise"])
+    eng.add_particles(scene["ptype"], scene["x"], scene["v"], scene["species"])
+    if scene.get("bonds") is not None:
+        eng.add_bonds(pots[scene["bonds"]["pot"]], scene["bonds"]["ij"])
+    if scene.get("angles") is not None:
+        eng.add_angles(pots[scene["angles"]["pot"]], scene["angles"]["ijk"])
+    return eng, pots
