@@ -157,10 +157,14 @@ struct System {
     int64_t time = 0;
     int dpd_noise = 0;
     uint64_t seed = 0;
-    // 1: evaluate the sweep bounds in unsigned arithmetic exactly as
-    // tf_runner_dopair.cpp:151,163 do (a negative projected gap wraps and
-    // ends the row); 0: the signed comparison the sweep is meant to make.
-    int sweep_unsigned = 0;
+    // Pair-task sweep (tf_runner_dopair.cpp:147-163):
+    //   0  no pruning: every particle pair of the two cells is tested;
+    //   1  sorted-projection pruning with signed bounds (exact when the cells
+    //      are cubic and span == 1, the only case where |shift| equals the
+    //      shift's projection on the sort axis);
+    //   2  the same pruning with the reference's unsigned arithmetic, where a
+    //      negative projected gap wraps around and ends the row.
+    int sweep_mode = 1;
     double epot = 0, epot_nb = 0, epot_bond = 0, epot_angle = 0, ekin = 0, virial = 0;
     std::atomic<int> toofast{0};
 
@@ -399,7 +403,22 @@ void do_pair(System &s, int ci_id, int cj_id, double &vir) {
     const unsigned *ip = &ci->sortlist[(size_t)ni * sid];
     const unsigned *jp = &cj->sortlist[(size_t)nj * sid];
     double epot = 0.0;
-    const bool uw = s.sweep_unsigned != 0;
+    if (s.sweep_mode == 0) {
+        double epot0 = 0.0;
+        for (Part &pi : ci->parts) {
+            const double pix[3] = {pi.x[0] - shift[0], pi.x[1] - shift[1], pi.x[2] - shift[2]};
+            for (Part &pj : cj->parts) {
+                double dx[3];
+                for (int k = 0; k < 3; k++) dx[k] = pix[k] - pj.x[k];
+                const double r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+                if (r2 > s.cutoff2) continue;
+                pair_body(s, pi, pj, dx, r2, epot0, vir);
+            }
+        }
+        ci->epot += epot0;
+        return;
+    }
+    const bool uw = s.sweep_mode == 2;
     auto gap_gt = [&](unsigned kj, unsigned ki) {
         if (uw) return (kj & 0xffff) + unsigned(dnshift) - (ki & 0xffff) > unsigned(dmaxdist);
         return int(kj & 0xffff) + dnshift - int(ki & 0xffff) > dmaxdist;
@@ -799,6 +818,9 @@ void *or_create(const double origin[3], const double dim[3], const int cells[3],
     s->potmap.assign((size_t)max_types * max_types, -1);
     s->fluxmap.assign((size_t)max_types * max_types, {});
     s->nspecies = nspecies;
+    const bool cubic = s->h[0] == s->h[1] && s->h[1] == s->h[2];
+    const bool span1 = s->span[0] == 1 && s->span[1] == 1 && s->span[2] == 1;
+    s->sweep_mode = (cubic && span1) ? 1 : 0;
     return s;
 }
 
@@ -985,10 +1007,13 @@ int or_energies(void *h, double out[6]) {
 
 long long or_time(void *h) { return static_cast<System *>(h)->time; }
 
-int or_set_sweep_unsigned(void *h, int on) {
-    static_cast<System *>(h)->sweep_unsigned = on;
+int or_set_sweep_mode(void *h, int mode) {
+    if (mode < 0 || mode > 2) return -1;
+    static_cast<System *>(h)->sweep_mode = mode;
     return 0;
 }
+
+int or_get_sweep_mode(void *h) { return static_cast<System *>(h)->sweep_mode; }
 
 void or_philox(unsigned c[4], unsigned k0, unsigned k1) { philox(c, k0, k1); }
 

@@ -34,7 +34,8 @@ _SIG = {
     "or_add_bonds": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip]),
     "or_add_angles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip]),
     "or_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_ulonglong]),
-    "or_set_sweep_unsigned": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "or_set_sweep_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "or_get_sweep_mode": (ctypes.c_int, [_vp]),
     "or_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "or_force": (ctypes.c_int, [_vp, ctypes.c_int]),
     "or_read": (ctypes.c_int, [_vp, _ip, _dp, _dp, _dp, _dp, _dp]),
@@ -153,8 +154,14 @@ class Oracle:
         ntasks = lib().or_geometry(self._h, cd, h, sp)
         return list(cd), list(h), list(sp), ntasks
 
-    def set_sweep_unsigned(self, on):
-        lib().or_set_sweep_unsigned(self._h, 1 if on else 0)
+    def set_sweep_mode(self, mode):
+        """0 exhaustive, 1 signed sorted sweep, 2 the reference's unsigned sweep."""
+        if lib().or_set_sweep_mode(self._h, mode) != 0:
+            raise ValueError(mode)
+
+    @property
+    def sweep_mode(self):
+        return lib().or_get_sweep_mode(self._h)
 
     def step(self, nsteps=1, threads=1):
         return lib().or_step(self._h, nsteps, threads)

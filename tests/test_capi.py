@@ -1,0 +1,91 @@
+"""The C-ABI library: loads, exports every symbol of include/tfg.h, and
+reports errors the way the reference's HRESULT convention does.  No device
+compute is called here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+E_FAIL = -2147467259
+E_INVALIDARG = -2147024809
+E_RANGE = -2147483637
+E_STATE = -2147418113
+E_DEVICE = -2147467258
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "tfg.h")).read()
+    return sorted(set(re.findall(r"\b(tfg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol(tf):
+    lib = ctypes.CDLL(tf.LIB_PATH)
+    syms = header_symbols()
+    assert len(syms) >= 40
+    for s in syms:
+        assert hasattr(lib, s), s
+    # the Python binding covers the whole header
+    assert set(syms) == set(tf.exported_symbols())
+
+
+def test_version(tf):
+    assert b"gfx950" in tf.lib().tfg_version()
+
+
+def test_error_codes(tf):
+    with pytest.raises(tf.TfgError) as ei:
+        tf.Engine(dim=[5, 5, 5], cells=[2, 2, 2], cutoff=2.0)   # fewer than 2*span+1 cells
+    assert ei.value.code == E_INVALIDARG
+    eng = tf.Engine(dim=[12, 12, 12], cells=[4, 4, 4], cutoff=3.0, nspecies=1)
+    a = eng.add_type("A", 1.0, 0.5)
+    pot = tf.Potential.morse(d=1.0, a=5.0, r0=1.0, min=0.5, max=3.0)
+    with pytest.raises(tf.TfgError) as ei:
+        eng.bind(pot, a, 3)
+    assert ei.value.code == E_RANGE
+    with pytest.raises(tf.TfgError) as ei:
+        eng.add_flux(tf.FLUX_FICK, a, a, 0, 2, 0.1)
+    assert ei.value.code == E_RANGE
+    with pytest.raises(tf.TfgError) as ei:
+        eng.add_particles(np.array([5], dtype=np.int32), np.array([[1.0, 1.0, 1.0]]))
+    assert ei.value.code == E_RANGE
+    eng.add_particles(np.array([a], dtype=np.int32), np.array([[1.0, 1.0, 1.0]]), species=np.array([[0.5]]))
+    with pytest.raises(tf.TfgError) as ei:
+        eng.step(1)   # the library steps only on the device
+    assert ei.value.code == E_STATE
+    assert "device" in str(ei.value)
+    with pytest.raises(tf.TfgError) as ei:
+        tf.Potential.lennard_jones_12_6(0.01, 3.0, 1.0, 1.0, tol=1e-14)
+    assert ei.value.code == E_RANGE
+    with pytest.raises(tf.TfgError):
+        eng.from_device()
+
+
+def test_to_device_fails_loudly_without_gpu(tf):
+    if tf.device_count() > 0:
+        pytest.skip("a HIP device is visible; the no-device path is tested on CPU hosts")
+    eng = tf.Engine(dim=[12, 12, 12], cells=[4, 4, 4], cutoff=3.0)
+    eng.add_type("A")
+    eng.add_particles(np.zeros(3, dtype=np.int32), np.random.default_rng(0).uniform(0, 12, (3, 3)))
+    with pytest.raises(tf.TfgError) as ei:
+        eng.to_device(tf.PREC_MIXED)
+    assert ei.value.code == E_DEVICE
+    assert not eng.on_device
+
+
+def test_geometry_and_insertion(tf):
+    eng = tf.Engine(dim=[20.0] * 3, cells=[29] * 3, cutoff=0.6)
+    cdim, h, span = eng.geometry()
+    assert cdim == [28] * 3   # floor(20 / (20 / 29)) == 28 in FP64
+    eng.add_type("A")
+    xs = np.array([[0.0, 19.99, 10.0], [20.0, 0.7142857, 5.0]])
+    eng.add_particles(np.zeros(2, dtype=np.int32), xs)
+    p = eng.particles()
+    for k in range(2):
+        x = np.clip(xs[k], 0.0, 20.0 * (1 - np.finfo(float).eps))
+        ind = (x * (1.0 / h[0])).astype(int)
+        assert p["cid"][k] == (ind[0] * 28 + ind[1]) * 28 + ind[2]
+        assert np.allclose(p["x"][k], x - ind * h[0], atol=1e-12)
