@@ -1,0 +1,311 @@
+// Bonded forces and the forward-Euler update (reads buffer A, writes B in
+// the same order), plus the FP64 energy and species-total reductions.
+//
+// Bonds follow bond_eval (tfBond.cpp:97-345), angles angle_eval
+// (tfAngle.cpp:90-384), gathered per particle over a static incidence list
+// so every particle's sum has a fixed order.  The update follows
+// cell_advance_forward_euler (tf_engine_advance.cpp:140-247): Newtonian or
+// overdamped Euler with the per-axis displacement clamp, cell migration by
+// delta = (x >= h) - (x < 0) and Fluxes_integrate (tfFlux.cpp:175-202) for
+// particles that stay in their cell.  The new cell counts for the next
+// binning are accumulated here, one atomic per run of equal cell ids.
+#include "ensemble.h"
+
+namespace tfg {
+
+constexpr int kEnergyTile = 4096;   // particles per energy-partial block
+
+__device__ inline int wrap_shift(int s) { return s > 1 ? -1 : (s < -1 ? 1 : s); }
+
+template <typename real>
+__device__ inline void eval_table_r(const PotDesc<real> &pd, const real *coef, real r, real *e, real *eff_out) {
+    const real *cc = coef + pd.coff + table_index<real>(pd, r) * 8;
+    const real x = (r - cc[0]) * cc[1];
+    real ee = cc[2] * x + cc[3];
+    real eff = cc[2];
+    for (int k = 4; k < 8; k++) {
+        eff = eff * x + ee;
+        ee = ee * x + cc[k];
+    }
+    *e = ee;
+    *eff_out = eff * cc[1];
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
+    using v4 = typename V<real>::v4;
+    const int S = P.S;
+    const int lane = threadIdx.x & 63;
+    const int wstart = (blockIdx.x * blockDim.x + threadIdx.x) - lane;
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = wstart; base < P.np; base += stride) {
+        const int i = base + lane;
+        const bool valid = i < P.np;
+        int newc = -1;
+        if (valid) {
+            const int4 mi = P.A.meta[i];
+            const int c = mi.z;
+            const int id = mi.y;
+            const int info = mi.x;
+            const SysDesc &sd = P.sys[P.cell_sys[c]];
+            v4 p = P.A.pos[i];
+            v4 v = P.A.vel[i];
+            v4 f = P.force[i];
+            int loc[3];
+            cell_loc(c - sd.cell_off, sd, loc);
+            const real h[3] = {real(sd.h[0]), real(sd.h[1]), real(sd.h[2])};
+
+            double eb = 0.0, ea = 0.0;
+            for (int q = P.inc_off[id]; q < P.inc_off[id + 1]; q++) {
+                const int ent = P.inc[q];
+                const int kind = ent >> 30;   // 0 bond, 1 angle
+                const int idx = ent & 0x3fffffff;
+                if (kind == 0) {
+                    const int4 b = P.bonds[idx];
+                    const int si = P.slot_of[b.x], sj = P.slot_of[b.y];
+                    const v4 xi = P.A.pos[si], xj = P.A.pos[sj];
+                    int li[3], lj[3];
+                    cell_loc(P.A.meta[si].z - sd.cell_off, sd, li);
+                    cell_loc(P.A.meta[sj].z - sd.cell_off, sd, lj);
+                    const real xiv[3] = {xi.x, xi.y, xi.z}, xjv[3] = {xj.x, xj.y, xj.z};
+                    real d[3];
+                    for (int k = 0; k < 3; k++) d[k] = (xiv[k] + h[k] * wrap_shift(li[k] - lj[k])) - xjv[k];
+                    const real r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    const PotDesc<real> &pd = P.pots[b.z];
+                    real rr = sqrt(r2);
+                    if (pd.flags & POT_SCALED)
+                        rr = rr / (xi.w + xj.w);
+                    else if (pd.flags & POT_SHIFTED)
+                        rr = rr - (xi.w + xj.w) + pd.r0p1;
+                    real rr2 = rr * rr;
+                    if (rr2 > pd.b * pd.b) continue;
+                    rr2 = rr2 > pd.a * pd.a ? rr2 : pd.a * pd.a;
+                    const real r = sqrt(rr2);
+                    real e, eff;
+                    eval_table_r<real>(pd, P.coef, r, &e, &eff);
+                    const real fr = eff / r;
+                    const real sgn = (id == b.x) ? real(-1) : real(1);
+                    f.x += sgn * fr * d[0];
+                    f.y += sgn * fr * d[1];
+                    f.z += sgn * fr * d[2];
+                    if (id == b.x) eb += static_cast<double>(e);
+                } else {
+                    const int4 a = P.angles[idx];
+                    const int si = P.slot_of[a.x], sj = P.slot_of[a.y], sk = P.slot_of[a.z];
+                    const v4 pi = P.A.pos[si], pj = P.A.pos[sj], pk = P.A.pos[sk];
+                    int li[3], lj[3], lk[3];
+                    cell_loc(P.A.meta[si].z - sd.cell_off, sd, li);
+                    cell_loc(P.A.meta[sj].z - sd.cell_off, sd, lj);
+                    cell_loc(P.A.meta[sk].z - sd.cell_off, sd, lk);
+                    const real piv[3] = {pi.x, pi.y, pi.z}, pjv[3] = {pj.x, pj.y, pj.z}, pkv[3] = {pk.x, pk.y, pk.z};
+                    real rji[3], rjk[3];
+                    for (int k = 0; k < 3; k++) {
+                        const real xi = piv[k] + wrap_shift(li[k] - lj[k]) * h[k];
+                        const real xk = pkv[k] + wrap_shift(lk[k] - lj[k]) * h[k];
+                        rji[k] = xi - pjv[k];
+                        rjk[k] = xk - pjv[k];
+                    }
+                    const real dprod = rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2];
+                    const real inji = real(1) / sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+                    const real injk = real(1) / sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+                    real ct = dprod * inji * injk;
+                    ct = ct < real(1) ? ct : real(1);
+                    ct = ct > real(-1) ? ct : real(-1);
+                    const PotDesc<real> &pd = P.pots[a.w];
+                    if (ct > pd.b) continue;
+                    real e, eff;
+                    eval_table_r<real>(pd, P.coef, ct > pd.a ? ct : pd.a, &e, &eff);
+                    real dxi[3], dxk[3];
+                    for (int k = 0; k < 3; k++) {
+                        dxi[k] = (rjk[k] * injk - ct * rji[k] * inji) * inji;
+                        dxk[k] = (rji[k] * inji - ct * rjk[k] * injk) * injk;
+                    }
+                    real g[3];
+                    if (id == a.x) {
+                        for (int k = 0; k < 3; k++) g[k] = -eff * dxi[k];
+                    } else if (id == a.z) {
+                        for (int k = 0; k < 3; k++) g[k] = -eff * dxk[k];
+                    } else {
+                        for (int k = 0; k < 3; k++) g[k] = eff * dxi[k] + eff * dxk[k];
+                        ea += static_cast<double>(e);
+                    }
+                    f.x += g[0];
+                    f.y += g[1];
+                    f.z += g[2];
+                }
+            }
+            P.force[i] = f;
+
+            const int ti = info & 0xffff;
+            const unsigned flags = static_cast<unsigned>(info) >> 16;
+            const TypeDesc &td = P.types[sd.type_off + ti];
+            if (P.energy) {
+                P.e_kin[i] = 0.5 * td.mass *
+                             (static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y +
+                              static_cast<double>(v.z) * v.z);
+                P.e_bond[i] = eb;
+                P.e_ang[i] = ea;
+            }
+            newc = c;
+            bool moved = false;
+            if (update) {
+                const bool frozen_all = (flags & (PFLAG_FROZEN_X | PFLAG_FROZEN_Y | PFLAG_FROZEN_Z)) ==
+                                        (PFLAG_FROZEN_X | PFLAG_FROZEN_Y | PFLAG_FROZEN_Z);
+                if (!(flags & PFLAG_CLUSTER) && !frozen_all) {
+                    // Same operation order as the reference, without contraction.
+#pragma clang fp contract(off)
+                    const real dt = real(sd.dt);
+                    const real im = P.A.imass[i];
+                    real xv[3] = {p.x, p.y, p.z}, vv[3] = {v.x, v.y, v.z};
+                    const real fv[3] = {f.x, f.y, f.z};
+                    int delta[3];
+                    for (int k = 0; k < 3; k++) {
+                        const real mask = (flags & (PFLAG_FROZEN_X << k)) ? real(0) : real(1);
+                        if (td.dynamics == DYN_NEWTONIAN) {
+                            const real vn = mask * (vv[k] + dt * fv[k] * im);
+                            const real mv = real(sd.maxv[k]);
+                            vv[k] = vn * vn <= mv * mv ? vn : (vn < real(0) ? -mv : mv);
+                            xv[k] += dt * vv[k];
+                        } else {
+                            real dxk = mask * (dt * fv[k] * im);
+                            const real mx = real(sd.maxx[k]);
+                            dxk = dxk * dxk <= mx * mx ? dxk : (dxk < real(0) ? -mx : mx);
+                            vv[k] = dxk / dt;
+                            xv[k] += dxk;
+                        }
+                        // x + foff >= h, x + foff < 0 in the stored frame
+                        delta[k] = (xv[k] >= real(sd.h[k] - sd.foff[k])) - (xv[k] < real(-sd.foff[k]));
+                    }
+                    if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
+                        moved = true;
+                        for (int k = 0; k < 3; k++) {
+                            if (xv[k] >= real(2.0 * sd.h[k] - sd.foff[k]) || xv[k] <= real(-sd.h[k] - sd.foff[k]))
+                                atomicOr(P.err, 1);
+                            xv[k] = real(static_cast<double>(xv[k]) - delta[k] * sd.h[k]);
+                            loc[k] = (loc[k] + delta[k] + sd.cdim[k]) % sd.cdim[k];
+                        }
+                        newc = sd.cell_off + (loc[0] * sd.cdim[1] + loc[1]) * sd.cdim[2] + loc[2];
+                    }
+                    p.x = xv[0];
+                    p.y = xv[1];
+                    p.z = xv[2];
+                    v.x = vv[0];
+                    v.y = vv[1];
+                    v.z = vv[2];
+                }
+            }
+            P.B.pos[i] = p;
+            P.B.vel[i] = v;
+            P.B.meta[i] = make_int4(info, id, newc, 0);
+            P.B.imass[i] = P.A.imass[i];
+            // Particles that changed cell skip this step's integration and keep
+            // their pending flux (Fluxes_integrate runs over the particles left
+            // in the cell, tf_engine_advance.cpp:430-436).
+            for (int k = 0; k < S; k++) {
+                const size_t o = (size_t)k * P.np + i;
+                real s = P.A.sp[o], qq = P.A.q[o];
+                if (update && !moved) {
+                    s = s + real(sd.dt_flux) * qq;
+                    s = s > real(0) ? s : real(0);
+                    qq = real(0);
+                }
+                P.B.sp[o] = s;
+                P.B.q[o] = qq;
+            }
+        }
+        int head, len;
+        wave_runs(newc, valid, lane, &head, &len);
+        if (valid && head == lane) atomicAdd(&P.cnt[newc], len);
+    }
+}
+
+// Energy partials: block (bx, s) sums particles [bx*tile, (bx+1)*tile) of
+// system s; part[(s*nbx + bx)*8 + k] for k = nonbond, virial, kinetic, bond,
+// angle, then up to 3 species totals... species use their own kernel.
+__global__ __launch_bounds__(256) void k_energy_part(const SysDesc *sys, const double *e_nb, const double *e_vir,
+                                                     const double *e_kin, const double *e_bond,
+                                                     const double *e_ang, double *part, int nbx) {
+    __shared__ double s[5][256];
+    const SysDesc &sd = sys[blockIdx.y];
+    const int b0 = blockIdx.x * kEnergyTile;
+    double a[5] = {0, 0, 0, 0, 0};
+    for (int t = threadIdx.x; t < kEnergyTile; t += 256) {
+        const int li = b0 + t;
+        if (li >= sd.np) break;
+        const int i = sd.p_off + li;
+        a[0] += e_nb[i];
+        a[1] += e_vir[i];
+        a[2] += e_kin[i];
+        a[3] += e_bond[i];
+        a[4] += e_ang[i];
+    }
+    for (int k = 0; k < 5; k++) s[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int k = 0; k < 5; k++) s[k][threadIdx.x] += s[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; k++) part[((size_t)blockIdx.y * nbx + blockIdx.x) * 5 + k] = s[k][0];
+}
+
+__global__ __launch_bounds__(256) void k_energy_final(const double *part, int nbx, double *out) {
+    __shared__ double s[5][256];
+    double a[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nbx; b += 256)
+        for (int k = 0; k < 5; k++) a[k] += part[((size_t)blockIdx.x * nbx + b) * 5 + k];
+    for (int k = 0; k < 5; k++) s[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int k = 0; k < 5; k++) s[k][threadIdx.x] += s[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; k++) out[5 * blockIdx.x + k] = s[k][0];
+}
+
+// Species totals per system in FP64 (conservation check of the flux pass).
+template <typename real>
+__global__ __launch_bounds__(256) void k_species_total(const SysDesc *sys, const real *sp, int np, int S, double *out) {
+    __shared__ double s[256];
+    const SysDesc &sd = sys[blockIdx.x];
+    for (int k = 0; k < S; k++) {
+        double a = 0;
+        for (int i = sd.p_off + threadIdx.x; i < sd.p_off + sd.np; i += 256) a += static_cast<double>(sp[(size_t)k * np + i]);
+        s[threadIdx.x] = a;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[blockIdx.x * kMaxSpecies + k] = s[0];
+        __syncthreads();
+    }
+}
+
+template <typename real> int run_advance(DeviceEnsemble *E, const KP<real> &P, int update) {
+    hipLaunchKernelGGL(k_advance<real>, dim3(E->stream_grid()), dim3(256), 0, E->stream, P, update);
+    return launch_error("advance");
+}
+
+template <typename real> int run_energy(DeviceEnsemble *E, const KP<real> &P) {
+    const int nbx = E->max_sys_blocks;
+    hipLaunchKernelGGL(k_energy_part, dim3(nbx, E->nsys), dim3(256), 0, E->stream, E->d_sys, E->e_nb, E->e_vir,
+                       E->e_kin, E->e_bond, E->e_ang, E->e_part, nbx);
+    hipLaunchKernelGGL(k_energy_final, dim3(E->nsys), dim3(256), 0, E->stream, E->e_part, nbx, E->d_energy);
+    if (E->S > 0)
+        hipLaunchKernelGGL(k_species_total<real>, dim3(E->nsys), dim3(256), 0, E->stream, E->d_sys, P.B.sp, E->np,
+                           E->S, E->d_spec_tot);
+    return launch_error("energy");
+}
+
+template int run_advance<float>(DeviceEnsemble *, const KP<float> &, int);
+template int run_advance<double>(DeviceEnsemble *, const KP<double> &, int);
+template int run_energy<float>(DeviceEnsemble *, const KP<float> &);
+template int run_energy<double>(DeviceEnsemble *, const KP<double> &);
+
+int energy_tile() { return kEnergyTile; }
+
+}  // namespace tfg

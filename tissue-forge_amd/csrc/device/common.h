@@ -1,0 +1,151 @@
+// Device-side data layout shared by the kernels of the engine_step module.
+//
+// An ensemble holds one or more independent systems in flat buffers.  The
+// particles of all systems are kept sorted by global cell id (system-major)
+// and, inside a cell, by particle id, so results never depend on scheduling.
+//
+// Stored positions are relative to the cell origin minus a per-system frame
+// offset: 0 on the FP64 path (the reference's cell-relative coordinates,
+// tfSpace.cpp:461-462) and half a cell on the FP32 path, where centring the
+// frame halves the magnitude, and so the rounding, of every coordinate.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tfg {
+
+template <typename T> struct V;
+template <> struct V<float> { using v4 = float4; };
+template <> struct V<double> { using v4 = double4; };
+
+constexpr int kWave = 64;
+constexpr int kMaxTypes = 8;          // type pairs cached in LDS by the pair kernel
+constexpr int kMaxSpecies = 8;        // TF_SIMD_SIZE
+constexpr int kMaxFluxEntries = 64;   // flux entries per system
+constexpr int kMaxSpanZ = 3;          // cells of cutoff along z per direction
+
+// Per-system constants.
+struct SysDesc {
+    int cdim[3];
+    int ncells, cell_off, p_off, np, ntypes;
+    int nstencil, stencil_off;   // columns (ox, oy) of the neighbour stencil
+    int span_z;                  // z extent of every column: -span_z .. span_z
+    int pot_off, type_off;
+    int flux_off;     // first (type pair) entry of this system in the flux map
+    int flux_first;   // first flux entry of this system; the map's offsets are relative to it
+    int nflux;
+    int coef_begin, coef_count;   // this system's pair-table coefficients
+    int dpd_noise;
+    uint32_t seed_lo, seed_hi;
+    long long time0;   // engine->time before the first device step
+    double h[3];
+    double foff[3];    // stored position = position relative to the cell origin - foff
+    double cutoff, cutoff2, dt, dt_flux, w_self, inv_sqrt_dt;
+    double maxx[3], maxv[3];
+};
+
+template <typename real> struct PotDesc {
+    int kind;
+    unsigned flags;
+    int n, coff;
+    real a, b, r0p1, al0, al1, al2;
+    real dalpha, dgamma, dsigma;
+};
+
+struct TypeDesc {
+    double mass;
+    int dynamics;
+    int pad;
+};
+
+template <typename real> struct FluxDesc {
+    int kind, type_a, type_b, index_a, index_b, pad;
+    real coef, half_decay, target, cutoff;
+};
+
+// Particle record, structure of arrays.
+template <typename real> struct PArrays {
+    typename V<real>::v4 *pos;   // x, y, z (stored frame); w = radius
+    typename V<real>::v4 *vel;   // vx, vy, vz, 0
+    int4 *meta;                  // type | flags << 16, global particle id, global cell id, 0
+    real *imass;
+    real *sp;                    // species values, S per particle (species-major)
+    real *q;                     // pending species flux
+};
+
+template <typename real> struct KP {
+    const SysDesc *sys;
+    const int *cell_sys;
+    const int *stencil;   // 2 ints (ox, oy) per stencil column
+    const PotDesc<real> *pots;
+    const int *potmap;
+    const real *coef;
+    const TypeDesc *types;
+    const FluxDesc<real> *flux;
+    const int *fluxmap;   // 2 ints per (system, type pair): first, count (system-relative)
+    PArrays<real> A, B;
+    typename V<real>::v4 *force;   // force + number density, A order
+    int *slot_of;
+    int *cnt, *fill, *start, *perm, *bsum;
+    int2 *tmp;
+    double *e_part;   // energy partials
+    double *e_nb, *e_vir, *e_kin, *e_bond, *e_ang;
+    const int *inc_off, *inc;
+    const int4 *bonds, *angles;
+    long long *step;
+    int *err;
+    int ncells, np, S, energy;
+    int step_ahead;   // 1 when forces are evaluated outside a step (time + 1)
+    int coef_lds;     // 1 when every system's pair tables fit the LDS cache
+};
+
+__device__ inline float as_real_bits(int v, float) { return __int_as_float(v); }
+__device__ inline double as_real_bits(int v, double) { return __longlong_as_double(static_cast<long long>(v)); }
+__device__ inline int real_bits_as_int(float v) { return __float_as_int(v); }
+__device__ inline int real_bits_as_int(double v) { return static_cast<int>(__double_as_longlong(v)); }
+
+__device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double rsq(double x) { return 1.0 / sqrt(x); }
+template <typename real> __device__ inline real eps_of();
+template <> __device__ inline float eps_of<float>() { return 1.1920928955078125e-07f; }
+template <> __device__ inline double eps_of<double>() { return 2.220446049250313e-16; }
+
+__device__ inline void cell_loc(int lc, const SysDesc &sd, int loc[3]) {
+    loc[0] = lc / (sd.cdim[1] * sd.cdim[2]);
+    loc[1] = (lc / sd.cdim[2]) % sd.cdim[1];
+    loc[2] = lc % sd.cdim[2];
+}
+
+// Interval of r in [a, b]: int(max(0, alpha0 + r*(alpha1 + r*alpha2)))
+// (tf_potential_eval.h:121-124).  In FP64 the widened table domain already
+// keeps this in [1, n] for a <= r <= b; the clamp makes FP32 rounding at the
+// two ends land on the same intervals instead of the left continuation
+// (interval 0) or past the last one.
+template <typename real>
+__device__ inline int table_index(const PotDesc<real> &pd, real r) {
+    const real t = pd.al0 + r * (pd.al1 + r * pd.al2);
+    int ind = static_cast<int>(t > real(0) ? t : real(0));
+    ind = ind < 1 ? 1 : ind;
+    return ind > pd.n ? pd.n : ind;
+}
+
+// Runs of equal keys among the lanes of a wave whose valid lanes form a
+// prefix.  Returns the head lane of this lane's run and, for head lanes, the
+// run length.
+__device__ inline void wave_runs(int key, bool valid, int lane, int *head, int *len) {
+    const int prev = __shfl_up(key, 1);
+    const bool is_head = valid && (lane == 0 || prev != key);
+    const unsigned long long heads = __ballot(is_head);
+    const unsigned long long vmask = __ballot(valid);
+    const int nvalid = __popcll(vmask);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long hb = heads & upto;
+    *head = hb ? 63 - __builtin_clzll(hb) : 0;
+    const unsigned long long after = lane == 63 ? 0ull : (heads >> (lane + 1));
+    const int next = after ? lane + 1 + __builtin_ctzll(after) : nvalid;
+    *len = next - lane;
+}
+
+}  // namespace tfg

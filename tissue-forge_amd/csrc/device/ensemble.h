@@ -1,0 +1,119 @@
+// Host-side state of one device ensemble and the launchers of each phase.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../host/device_api.h"
+#include "../host/engine.h"
+#include "../host/errors.h"
+#include "common.h"
+
+namespace tfg {
+
+#define TFG_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return set_error(TFG_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct Buffers {
+    std::vector<void *> ptrs;
+    ~Buffers() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <typename T> int alloc(T **out, size_t n) {
+        void *p = nullptr;
+        if (n == 0) n = 1;
+        TFG_HIP(hipMalloc(&p, n * sizeof(T)));
+        ptrs.push_back(p);
+        *out = static_cast<T *>(p);
+        return TFG_OK;
+    }
+};
+
+// Launch shape of the pair kernel and the streaming kernels.
+struct LaunchShape {
+    int nonbond_blocks = 0;   // persistent workgroups of the pair kernel (0 = 8 per CU)
+    int stream_threads = 256;
+    int stream_blocks = 0;
+};
+
+struct DeviceEnsemble {
+    int precision = PREC_MIXED;
+    int device = 0;
+    int cus = 256;
+    std::vector<Engine *> engines;
+    std::vector<SysDesc> sys;
+    int nsys = 0, np = 0, ncells = 0, S = 0;
+    int max_sys_blocks = 1;   // energy partial blocks per system
+    bool has_dpd = false, has_flux = false, has_radius = false;
+    int coef_lds_bytes = 0;   // table bytes cached in LDS by the pair kernel (0: read from global)
+    int nonbond_lds = 0;      // dynamic LDS of the pair kernel (pair descriptors + tables)
+    LaunchShape shape;
+    Buffers buf;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8];
+    int nev = 0;
+    float last_ms = 0.0f;
+    double kernel_ms[3] = {0, 0, 0};
+    hipGraphExec_t graph = nullptr;
+    bool profile = false;
+    std::vector<hipEvent_t> prof_ev;
+    double prof_ms[3] = {0, 0, 0};
+    int prof_steps = 0;
+
+    SysDesc *d_sys = nullptr;
+    int *d_cell_sys = nullptr, *d_stencil = nullptr, *d_potmap = nullptr, *d_fluxmap = nullptr;
+    void *d_pots = nullptr, *d_coef = nullptr, *d_flux = nullptr;
+    TypeDesc *d_types = nullptr;
+    void *pos[2] = {nullptr, nullptr}, *vel[2] = {nullptr, nullptr}, *imass[2] = {nullptr, nullptr};
+    void *sp[2] = {nullptr, nullptr}, *qq[2] = {nullptr, nullptr};
+    int4 *meta[2] = {nullptr, nullptr};
+    void *force = nullptr;
+    int *slot_of = nullptr, *cnt = nullptr, *fill = nullptr, *start = nullptr, *perm = nullptr, *bsum = nullptr;
+    int2 *tmp = nullptr;
+    double *e_part = nullptr, *e_nb = nullptr, *e_vir = nullptr, *e_kin = nullptr, *e_bond = nullptr,
+           *e_ang = nullptr;
+    double *d_energy = nullptr, *d_spec_tot = nullptr;
+    int *inc_off = nullptr, *inc = nullptr;
+    int4 *d_bonds = nullptr, *d_angles = nullptr;
+    long long *d_step = nullptr;
+    int *d_err = nullptr;
+
+    ~DeviceEnsemble() {
+        for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (int k = 0; k < nev; k++) (void)hipEventDestroy(ev[k]);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int stream_grid() const {
+        const int want = (np + shape.stream_threads - 1) / shape.stream_threads;
+        const int cap = shape.stream_blocks > 0 ? shape.stream_blocks : cus * 8;
+        return want < 1 ? 1 : (want < cap ? want : cap);
+    }
+    int nonbond_grid() const {
+        const int cap = shape.nonbond_blocks > 0 ? shape.nonbond_blocks : cus * 8;
+        return ncells < cap ? ncells : cap;
+    }
+};
+
+int launch_error(const char *what);
+
+template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy);
+
+// Binning: scan of the cell counts, placement, per-cell order, permutation
+// of buffer B into buffer A.  Cell counts come from the previous advance.
+template <typename real> int run_bin(DeviceEnsemble *E, const KP<real> &P);
+// Counts particles per cell from buffer B (after an upload).
+int run_count(DeviceEnsemble *E);
+template <typename real> int run_nonbond(DeviceEnsemble *E, const KP<real> &P);
+template <typename real> int nonbond_lds_bytes(int max_types, int max_coef, bool coef_lds);
+template <typename real> int run_advance(DeviceEnsemble *E, const KP<real> &P, int update);
+template <typename real> int run_energy(DeviceEnsemble *E, const KP<real> &P);
+
+}  // namespace tfg

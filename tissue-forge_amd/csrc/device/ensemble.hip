@@ -1,0 +1,687 @@
+// Host side of the device module: upload (engine_toCUDA role), step
+// sequencing and graph replay, readback (engine_fromCUDA role).
+#include "ensemble.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+namespace tfg {
+
+int energy_tile();
+
+int launch_error(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(TFG_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return TFG_OK;
+}
+
+template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy) {
+    using v4 = typename V<real>::v4;
+    KP<real> P;
+    P.sys = E->d_sys;
+    P.cell_sys = E->d_cell_sys;
+    P.stencil = E->d_stencil;
+    P.pots = static_cast<const PotDesc<real> *>(E->d_pots);
+    P.potmap = E->d_potmap;
+    P.coef = static_cast<const real *>(E->d_coef);
+    P.types = E->d_types;
+    P.flux = static_cast<const FluxDesc<real> *>(E->d_flux);
+    P.fluxmap = E->d_fluxmap;
+    PArrays<real> *arr[2] = {&P.A, &P.B};
+    for (int s = 0; s < 2; s++) {
+        arr[s]->pos = static_cast<v4 *>(E->pos[s]);
+        arr[s]->vel = static_cast<v4 *>(E->vel[s]);
+        arr[s]->meta = E->meta[s];
+        arr[s]->imass = static_cast<real *>(E->imass[s]);
+        arr[s]->sp = static_cast<real *>(E->sp[s]);
+        arr[s]->q = static_cast<real *>(E->qq[s]);
+    }
+    P.force = static_cast<v4 *>(E->force);
+    P.slot_of = E->slot_of;
+    P.cnt = E->cnt;
+    P.fill = E->fill;
+    P.start = E->start;
+    P.perm = E->perm;
+    P.bsum = E->bsum;
+    P.tmp = E->tmp;
+    P.e_part = E->e_part;
+    P.e_nb = E->e_nb;
+    P.e_vir = E->e_vir;
+    P.e_kin = E->e_kin;
+    P.e_bond = E->e_bond;
+    P.e_ang = E->e_ang;
+    P.inc_off = E->inc_off;
+    P.inc = E->inc;
+    P.bonds = E->d_bonds;
+    P.angles = E->d_angles;
+    P.step = E->d_step;
+    P.err = E->d_err;
+    P.ncells = E->ncells;
+    P.np = E->np;
+    P.S = E->S;
+    P.energy = energy;
+    P.step_ahead = 0;
+    P.coef_lds = E->coef_lds_bytes > 0 ? 1 : 0;
+    return P;
+}
+
+__global__ void k_tick(long long *step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+// One step: tick, bin, nonbond, advance (energy pass when requested).
+template <typename real> static int enqueue_step(DeviceEnsemble *E, bool energy, hipEvent_t *ev) {
+    KP<real> P = make_params<real>(E, energy ? 1 : 0);
+    int rc;
+    if (ev) TFG_HIP(hipEventRecord(ev[0], E->stream));
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, E->stream, E->d_step);
+    if ((rc = run_bin<real>(E, P)) != TFG_OK) return rc;
+    if (ev) TFG_HIP(hipEventRecord(ev[1], E->stream));
+    if ((rc = run_nonbond<real>(E, P)) != TFG_OK) return rc;
+    if (ev) TFG_HIP(hipEventRecord(ev[2], E->stream));
+    if ((rc = run_advance<real>(E, P, 1)) != TFG_OK) return rc;
+    if (ev) TFG_HIP(hipEventRecord(ev[3], E->stream));
+    if (energy && (rc = run_energy<real>(E, P)) != TFG_OK) return rc;
+    return TFG_OK;
+}
+
+int device_select(int ordinal) {
+    int n = 0;
+    TFG_HIP(hipGetDeviceCount(&n));
+    if (ordinal < 0 || ordinal >= n) return set_error(TFG_E_RANGE, "device ordinal out of range");
+    TFG_HIP(hipSetDevice(ordinal));
+    return TFG_OK;
+}
+
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+template <typename real> static int upload(DeviceEnsemble *E);
+
+int ensemble_create(DeviceEnsemble **out, Engine *const *engines, int n, int precision) {
+    if (n < 1) return set_error(TFG_E_INVALIDARG, "empty ensemble");
+    if (precision != PREC_MIXED && precision != PREC_FP64) return set_error(TFG_E_INVALIDARG, "unknown precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_error(TFG_E_DEVICE, "no HIP device available");
+    for (int k = 0; k < n; k++) {
+        if (!engines[k]) return set_error(TFG_E_INVALIDARG, "null engine");
+        if (engines[k]->device) return set_error(TFG_E_STATE, "engine already on a device");
+        for (int j = 0; j < k; j++)
+            if (engines[j] == engines[k]) return set_error(TFG_E_INVALIDARG, "engine listed twice");
+    }
+    DeviceEnsemble *E = new DeviceEnsemble();
+    E->precision = precision;
+    E->engines.assign(engines, engines + n);
+    int rc = precision == PREC_MIXED ? upload<float>(E) : upload<double>(E);
+    if (rc != TFG_OK) {
+        delete E;
+        return rc;
+    }
+    for (int k = 0; k < n; k++) {
+        engines[k]->device = E;
+        engines[k]->device_index = k;
+    }
+    *out = E;
+    return TFG_OK;
+}
+
+template <typename real> static int upload(DeviceEnsemble *E) {
+    using v4 = typename V<real>::v4;
+    TFG_HIP(hipGetDevice(&E->device));
+    TFG_HIP(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+    for (int k = 0; k < 8; k++) TFG_HIP(hipEventCreate(&E->ev[k]));
+    E->nev = 8;
+    hipDeviceProp_t prop;
+    TFG_HIP(hipGetDeviceProperties(&prop, E->device));
+    E->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    const int nsys = static_cast<int>(E->engines.size());
+    E->nsys = nsys;
+    // FP32 positions are stored relative to the cell centre.
+    const bool centred = std::is_same<real, float>::value;
+
+    std::vector<SysDesc> sys(nsys);
+    std::vector<int> cell_sys, stencil, potmap, fluxmap;
+    std::vector<PotDesc<real>> pots;
+    std::vector<real> coef;
+    std::vector<TypeDesc> types;
+    std::vector<FluxDesc<real>> flux;
+    int S = 0;
+    for (Engine *e : E->engines) S = std::max(S, e->nspecies);
+    if (S > kMaxSpecies) return set_error(TFG_E_RANGE, "too many species");
+    E->S = S;
+
+    // Registry id -> device potential index, per system.
+    std::vector<std::vector<int>> remaps(nsys);
+    int np = 0, nc = 0, max_coef = 0;
+    for (int s = 0; s < nsys; s++) {
+        Engine &e = *E->engines[s];
+        SysDesc &d = sys[s];
+        std::memset(&d, 0, sizeof(d));
+        const int nt = static_cast<int>(e.types.size());
+        if (nt > kMaxTypes) return set_error(TFG_E_RANGE, "the device module supports at most 8 particle types per system");
+        for (int k = 0; k < 3; k++) {
+            d.cdim[k] = e.geom.cdim[k];
+            d.h[k] = e.geom.h[k];
+            d.foff[k] = centred ? 0.5 * e.geom.h[k] : 0.0;
+            d.maxx[k] = e.geom.h[k] * e.max_dist_fraction;
+            d.maxv[k] = d.maxx[k] / e.dt;
+        }
+        d.ncells = e.geom.ncells;
+        d.cell_off = nc;
+        d.p_off = np;
+        d.np = static_cast<int>(e.parts.size());
+        d.ntypes = nt;
+        // Neighbour stencil as columns (ox, oy) spanning z in [-span_z, span_z]:
+        // a column is kept when its closest cell is inside the cutoff (the
+        // task cull of tfSpace.cpp:949-954 restricted to x and y); cells of a
+        // kept column that the full cull would drop are beyond the cutoff and
+        // only cost distance tests.
+        if (e.geom.span[2] > kMaxSpanZ)
+            return set_error(TFG_E_RANGE, "the cutoff spans more than 3 cells along z");
+        d.span_z = e.geom.span[2];
+        d.stencil_off = static_cast<int>(stencil.size() / 2);
+        for (int ox = -e.geom.span[0]; ox <= e.geom.span[0]; ox++)
+            for (int oy = -e.geom.span[1]; oy <= e.geom.span[1]; oy++) {
+                const double lx = e.geom.h[0] * std::fmax(std::abs(ox) - 1, 0);
+                const double ly = e.geom.h[1] * std::fmax(std::abs(oy) - 1, 0);
+                if (lx * lx + ly * ly > e.geom.cutoff2) continue;
+                stencil.push_back(ox);
+                stencil.push_back(oy);
+            }
+        d.nstencil = static_cast<int>(stencil.size() / 2) - d.stencil_off;
+        d.cutoff = e.geom.cutoff;
+        d.cutoff2 = e.geom.cutoff2;
+        d.dt = e.dt;
+        d.dt_flux = e.dt_flux;
+        d.w_self = 1.0 / (M_PI * e.geom.cutoff * e.geom.cutoff * e.geom.cutoff);   // W(0, rc)
+        d.inv_sqrt_dt = 1.0 / std::sqrt(e.dt);
+        d.dpd_noise = e.dpd_noise;
+        d.seed_lo = static_cast<uint32_t>(e.dpd_seed);
+        d.seed_hi = static_cast<uint32_t>(e.dpd_seed >> 32);
+        d.time0 = e.time;
+        std::vector<int> &remap = remaps[s];
+        remap.assign(e.pots.size(), -1);
+        auto add_pot = [&](int pid) -> int {
+            if (remap[pid] >= 0) return remap[pid];
+            const Potential &p = *e.pots[pid];
+            PotDesc<real> pd{};
+            pd.kind = static_cast<int>(p.kind);
+            pd.flags = p.flags;
+            pd.n = p.n;
+            pd.coff = static_cast<int>(coef.size());
+            pd.a = real(p.a);
+            pd.b = real(p.b);
+            pd.r0p1 = real(p.r0_plusone);
+            pd.al0 = real(p.alpha[0]);
+            pd.al1 = real(p.alpha[1]);
+            pd.al2 = real(p.alpha[2]);
+            pd.dalpha = real(p.dpd_alpha);
+            pd.dgamma = real(p.dpd_gamma);
+            pd.dsigma = real(p.dpd_sigma);
+            for (double cval : p.c) coef.push_back(real(cval));
+            remap[pid] = static_cast<int>(pots.size());
+            pots.push_back(pd);
+            return remap[pid];
+        };
+        d.pot_off = static_cast<int>(potmap.size());
+        d.coef_begin = static_cast<int>(coef.size());
+        for (int a = 0; a < nt; a++)
+            for (int b = 0; b < nt; b++) {
+                const int pid = e.potmap[a * e.max_types + b];
+                if (pid >= 0) {
+                    const Potential &p = *e.pots[pid];
+                    if (p.kind == KIND_DPD) E->has_dpd = true;
+                    if (p.flags & (POT_SHIFTED | POT_SCALED)) E->has_radius = true;
+                }
+                potmap.push_back(pid >= 0 ? add_pot(pid) : -1);
+            }
+        d.coef_count = static_cast<int>(coef.size()) - d.coef_begin;
+        max_coef = std::max(max_coef, d.coef_count);
+        d.type_off = static_cast<int>(types.size());
+        for (int a = 0; a < nt; a++) types.push_back({e.types[a].mass, e.types[a].dynamics, 0});
+        d.flux_off = static_cast<int>(fluxmap.size() / 2);
+        d.flux_first = static_cast<int>(flux.size());
+        for (int a = 0; a < nt; a++)
+            for (int b = 0; b < nt; b++) {
+                const int first = static_cast<int>(flux.size()) - d.flux_first;
+                for (const FluxEntry &fe : e.fluxes) {
+                    if (!((fe.type_a == a && fe.type_b == b) || (fe.type_a == b && fe.type_b == a))) continue;
+                    FluxDesc<real> fd{};
+                    fd.kind = fe.kind;
+                    fd.type_a = fe.type_a;
+                    fd.type_b = fe.type_b;
+                    fd.index_a = fe.index_a;
+                    fd.index_b = fe.index_b;
+                    fd.coef = real(fe.coef);
+                    fd.half_decay = real(fe.decay / 2.0);
+                    fd.target = real(fe.target);
+                    fd.cutoff = real(fe.cutoff);
+                    flux.push_back(fd);
+                    E->has_flux = true;
+                }
+                fluxmap.push_back(first);
+                fluxmap.push_back(static_cast<int>(flux.size()) - d.flux_first - first);
+            }
+        d.nflux = static_cast<int>(flux.size()) - d.flux_first;
+        if (d.nflux > kMaxFluxEntries) return set_error(TFG_E_RANGE, "too many flux entries in one system");
+        for (int c = 0; c < d.ncells; c++) cell_sys.push_back(s);
+        for (const BondRec &b : e.bonds) add_pot(b.pot);
+        for (const AngleRec &a : e.angles) add_pot(a.pot);
+        np += d.np;
+        nc += d.ncells;
+    }
+    E->np = np;
+    E->ncells = nc;
+    E->sys = sys;
+    int max_np = 1;
+    for (const SysDesc &d : sys) max_np = std::max(max_np, d.np);
+    E->max_sys_blocks = (max_np + energy_tile() - 1) / energy_tile();
+    // Pair tables of a system are cached in LDS when the largest fits 32 KiB.
+    const size_t cbytes = static_cast<size_t>(max_coef) * sizeof(real);
+    E->coef_lds_bytes = (cbytes > 0 && cbytes <= 32768) ? static_cast<int>(cbytes) : 0;
+    int max_nt = 1;
+    for (const SysDesc &d : sys) max_nt = std::max(max_nt, d.ntypes);
+    E->nonbond_lds = nonbond_lds_bytes<real>(max_nt, max_coef, E->coef_lds_bytes > 0);
+
+    // Particles in id order (slot == global id) into buffer B, and the bonded
+    // incidence lists (CSR by global particle id).
+    std::vector<v4> hpos(np), hvel(np);
+    std::vector<real> him(np), hsp((size_t)np * S, real(0)), hq((size_t)np * S, real(0));
+    std::vector<int4> hmeta(np);
+    std::vector<int> hslot(np), deg(np + 1, 0), inc;
+    std::vector<int4> hb, ha;
+    for (int s = 0; s < nsys; s++) {
+        Engine &e = *E->engines[s];
+        const SysDesc &d = sys[s];
+        for (int p = 0; p < d.np; p++) {
+            const HostParticle &hp = e.parts[p];
+            const int g = d.p_off + p;
+            hpos[g].x = real(hp.x[0] - d.foff[0]);
+            hpos[g].y = real(hp.x[1] - d.foff[1]);
+            hpos[g].z = real(hp.x[2] - d.foff[2]);
+            hpos[g].w = real(hp.radius);
+            hvel[g].x = real(hp.v[0]);
+            hvel[g].y = real(hp.v[1]);
+            hvel[g].z = real(hp.v[2]);
+            hvel[g].w = real(0);
+            him[g] = real(hp.imass);
+            hmeta[g] = make_int4((hp.type & 0xffff) | static_cast<int>(hp.flags << 16), g, d.cell_off + hp.cid, 0);
+            hslot[g] = g;
+            for (int k = 0; k < e.nspecies; k++) {
+                hsp[(size_t)k * np + g] = real(e.species[(size_t)p * e.nspecies + k]);
+                hq[(size_t)k * np + g] = real(e.species_q[(size_t)p * e.nspecies + k]);
+            }
+        }
+        const std::vector<int> &remap = remaps[s];
+        for (const BondRec &b : e.bonds) {
+            hb.push_back(make_int4(d.p_off + b.i, d.p_off + b.j, remap[b.pot], s));
+            deg[d.p_off + b.i + 1]++;
+            deg[d.p_off + b.j + 1]++;
+        }
+        for (const AngleRec &a : e.angles) {
+            ha.push_back(make_int4(d.p_off + a.i, d.p_off + a.j, d.p_off + a.k, remap[a.pot]));
+            deg[d.p_off + a.i + 1]++;
+            deg[d.p_off + a.j + 1]++;
+            deg[d.p_off + a.k + 1]++;
+        }
+    }
+    for (int g = 0; g < np; g++) deg[g + 1] += deg[g];
+    inc.assign(deg[np], 0);
+    {
+        std::vector<int> fillp(deg.begin(), deg.end() - 1);
+        for (size_t b = 0; b < hb.size(); b++) {
+            inc[fillp[hb[b].x]++] = static_cast<int>(b);
+            inc[fillp[hb[b].y]++] = static_cast<int>(b);
+        }
+        for (size_t a = 0; a < ha.size(); a++) {
+            const int ent = (1 << 30) | static_cast<int>(a);
+            inc[fillp[ha[a].x]++] = ent;
+            inc[fillp[ha[a].y]++] = ent;
+            inc[fillp[ha[a].z]++] = ent;
+        }
+    }
+
+    Buffers &B = E->buf;
+    int rc;
+#define ALLOC(ptr, T, n)                                            \
+    do {                                                            \
+        T *p_ = nullptr;                                            \
+        if ((rc = B.alloc<T>(&p_, (n))) != TFG_OK) return rc;       \
+        ptr = p_;                                                   \
+    } while (0)
+    const int nb_scan = (nc + 4095) / 4096;
+    ALLOC(E->d_sys, SysDesc, nsys);
+    ALLOC(E->d_cell_sys, int, nc);
+    ALLOC(E->d_stencil, int, stencil.size());
+    ALLOC(E->d_potmap, int, potmap.size());
+    ALLOC(E->d_fluxmap, int, fluxmap.size());
+    ALLOC(E->d_pots, PotDesc<real>, pots.size());
+    ALLOC(E->d_coef, real, coef.size());
+    ALLOC(E->d_flux, FluxDesc<real>, flux.size());
+    ALLOC(E->d_types, TypeDesc, types.size());
+    for (int s = 0; s < 2; s++) {
+        ALLOC(E->pos[s], v4, np);
+        ALLOC(E->vel[s], v4, np);
+        ALLOC(E->meta[s], int4, np);
+        ALLOC(E->imass[s], real, np);
+        ALLOC(E->sp[s], real, (size_t)np * S);
+        ALLOC(E->qq[s], real, (size_t)np * S);
+    }
+    ALLOC(E->force, v4, np);
+    ALLOC(E->slot_of, int, np);
+    ALLOC(E->cnt, int, nc);
+    ALLOC(E->fill, int, nc);
+    ALLOC(E->start, int, nc + 1);
+    ALLOC(E->bsum, int, nb_scan + 1);
+    ALLOC(E->perm, int, np);
+    ALLOC(E->tmp, int2, np);
+    ALLOC(E->e_nb, double, np);
+    ALLOC(E->e_vir, double, np);
+    ALLOC(E->e_kin, double, np);
+    ALLOC(E->e_bond, double, np);
+    ALLOC(E->e_ang, double, np);
+    ALLOC(E->e_part, double, (size_t)5 * E->max_sys_blocks * nsys);
+    ALLOC(E->d_energy, double, 5 * nsys);
+    ALLOC(E->d_spec_tot, double, kMaxSpecies * nsys);
+    ALLOC(E->d_step, long long, 1);
+    ALLOC(E->d_err, int, 1);
+    ALLOC(E->inc_off, int, np + 1);
+    ALLOC(E->inc, int, inc.size());
+    ALLOC(E->d_bonds, int4, hb.size());
+    ALLOC(E->d_angles, int4, ha.size());
+#undef ALLOC
+
+    auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (bytes == 0) return TFG_OK;
+        TFG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return TFG_OK;
+    };
+    if ((rc = H2D(E->d_sys, sys.data(), sizeof(SysDesc) * nsys))) return rc;
+    if ((rc = H2D(E->d_cell_sys, cell_sys.data(), sizeof(int) * cell_sys.size()))) return rc;
+    if ((rc = H2D(E->d_stencil, stencil.data(), sizeof(int) * stencil.size()))) return rc;
+    if ((rc = H2D(E->d_potmap, potmap.data(), sizeof(int) * potmap.size()))) return rc;
+    if ((rc = H2D(E->d_fluxmap, fluxmap.data(), sizeof(int) * fluxmap.size()))) return rc;
+    if ((rc = H2D(E->d_pots, pots.data(), sizeof(PotDesc<real>) * pots.size()))) return rc;
+    if ((rc = H2D(E->d_coef, coef.data(), sizeof(real) * coef.size()))) return rc;
+    if ((rc = H2D(E->d_flux, flux.data(), sizeof(FluxDesc<real>) * flux.size()))) return rc;
+    if ((rc = H2D(E->d_types, types.data(), sizeof(TypeDesc) * types.size()))) return rc;
+    if ((rc = H2D(E->pos[1], hpos.data(), sizeof(v4) * np))) return rc;
+    if ((rc = H2D(E->vel[1], hvel.data(), sizeof(v4) * np))) return rc;
+    if ((rc = H2D(E->imass[1], him.data(), sizeof(real) * np))) return rc;
+    if ((rc = H2D(E->sp[1], hsp.data(), sizeof(real) * hsp.size()))) return rc;
+    if ((rc = H2D(E->qq[1], hq.data(), sizeof(real) * hq.size()))) return rc;
+    if ((rc = H2D(E->meta[1], hmeta.data(), sizeof(int4) * np))) return rc;
+    if ((rc = H2D(E->slot_of, hslot.data(), sizeof(int) * np))) return rc;
+    if ((rc = H2D(E->inc_off, deg.data(), sizeof(int) * (np + 1)))) return rc;
+    if ((rc = H2D(E->inc, inc.data(), sizeof(int) * inc.size()))) return rc;
+    if ((rc = H2D(E->d_bonds, hb.data(), sizeof(int4) * hb.size()))) return rc;
+    if ((rc = H2D(E->d_angles, ha.data(), sizeof(int4) * ha.size()))) return rc;
+    // Zeroed state (outside any graph capture).
+    TFG_HIP(hipMemset(E->d_step, 0, sizeof(long long)));
+    TFG_HIP(hipMemset(E->d_err, 0, sizeof(int)));
+    TFG_HIP(hipMemset(E->cnt, 0, sizeof(int) * nc));
+    TFG_HIP(hipMemset(E->fill, 0, sizeof(int) * nc));
+    TFG_HIP(hipMemset(E->force, 0, sizeof(v4) * (size_t)np));
+    for (double *p : {E->e_nb, E->e_vir, E->e_kin, E->e_bond, E->e_ang})
+        TFG_HIP(hipMemset(p, 0, sizeof(double) * (size_t)np));
+    TFG_HIP(hipMemset(E->d_energy, 0, sizeof(double) * 5 * nsys));
+    TFG_HIP(hipMemset(E->d_spec_tot, 0, sizeof(double) * kMaxSpecies * nsys));
+    TFG_HIP(hipDeviceSynchronize());
+    if ((rc = run_count(E)) != TFG_OK) return rc;
+    TFG_HIP(hipStreamSynchronize(E->stream));
+    return TFG_OK;
+}
+
+int ensemble_destroy(DeviceEnsemble *E, bool download) {
+    if (!E) return TFG_OK;
+    int rc = TFG_OK;
+    if (download) rc = ensemble_download(E);
+    for (Engine *e : E->engines) {
+        e->device = nullptr;
+        e->device_index = -1;
+    }
+    delete E;
+    return rc;
+}
+
+template <typename real> static int download_t(DeviceEnsemble *E, int only) {
+    using v4 = typename V<real>::v4;
+    TFG_HIP(hipStreamSynchronize(E->stream));
+    const int np = E->np, S = E->S;
+    std::vector<v4> hpos(np), hvel(np), hf(np);
+    std::vector<int4> hmeta(np);
+    std::vector<int> hslot(np);
+    std::vector<real> hsp((size_t)np * S), hq((size_t)np * S);
+    // Current state: buffer B (the state after the last update, in A's order).
+    TFG_HIP(hipMemcpy(hpos.data(), E->pos[1], sizeof(v4) * np, hipMemcpyDeviceToHost));
+    TFG_HIP(hipMemcpy(hvel.data(), E->vel[1], sizeof(v4) * np, hipMemcpyDeviceToHost));
+    TFG_HIP(hipMemcpy(hmeta.data(), E->meta[1], sizeof(int4) * np, hipMemcpyDeviceToHost));
+    TFG_HIP(hipMemcpy(hf.data(), E->force, sizeof(v4) * np, hipMemcpyDeviceToHost));
+    TFG_HIP(hipMemcpy(hslot.data(), E->slot_of, sizeof(int) * np, hipMemcpyDeviceToHost));
+    if (S > 0) {
+        TFG_HIP(hipMemcpy(hsp.data(), E->sp[1], sizeof(real) * hsp.size(), hipMemcpyDeviceToHost));
+        TFG_HIP(hipMemcpy(hq.data(), E->qq[1], sizeof(real) * hq.size(), hipMemcpyDeviceToHost));
+    }
+    long long steps = 0;
+    TFG_HIP(hipMemcpy(&steps, E->d_step, sizeof(steps), hipMemcpyDeviceToHost));
+    for (int s = 0; s < E->nsys; s++) {
+        if (only >= 0 && s != only) continue;
+        Engine &e = *E->engines[s];
+        const SysDesc &d = E->sys[s];
+        for (int p = 0; p < d.np; p++) {
+            const int slot = hslot[d.p_off + p];
+            HostParticle &hp = e.parts[p];
+            hp.x[0] = static_cast<double>(hpos[slot].x) + d.foff[0];
+            hp.x[1] = static_cast<double>(hpos[slot].y) + d.foff[1];
+            hp.x[2] = static_cast<double>(hpos[slot].z) + d.foff[2];
+            hp.v[0] = hvel[slot].x;
+            hp.v[1] = hvel[slot].y;
+            hp.v[2] = hvel[slot].z;
+            hp.f[0] = hf[slot].x;
+            hp.f[1] = hf[slot].y;
+            hp.f[2] = hf[slot].z;
+            hp.f[3] = hf[slot].w;
+            hp.cid = hmeta[slot].z - d.cell_off;
+            hp.flags = static_cast<uint32_t>(hmeta[slot].x) >> 16;
+            for (int k = 0; k < e.nspecies; k++) {
+                e.species[(size_t)p * e.nspecies + k] = hsp[(size_t)k * np + slot];
+                e.species_q[(size_t)p * e.nspecies + k] = hq[(size_t)k * np + slot];
+            }
+        }
+        e.time = d.time0 + steps;
+        Energies en;
+        int rc = ensemble_energies(E, s, &en);
+        if (rc != TFG_OK) return rc;
+        e.energies = en;
+    }
+    return TFG_OK;
+}
+
+int ensemble_download(DeviceEnsemble *E) {
+    return E->precision == PREC_MIXED ? download_t<float>(E, -1) : download_t<double>(E, -1);
+}
+
+int ensemble_download_one(DeviceEnsemble *E, int sys) {
+    return E->precision == PREC_MIXED ? download_t<float>(E, sys) : download_t<double>(E, sys);
+}
+
+template <typename real> static int step_t(DeviceEnsemble *E, int nsteps, bool use_graph) {
+    int rc;
+    TFG_HIP(hipEventRecord(E->ev[4], E->stream));
+    if (use_graph && nsteps > 1) {
+        // One energy-free step is captured once and replayed; the last step
+        // runs eagerly with the energy pass.
+        if (!E->graph) {
+            TFG_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_step<real>(E, false, nullptr);
+            hipGraph_t captured = nullptr;
+            hipError_t ce = hipStreamEndCapture(E->stream, &captured);
+            if (rc != TFG_OK) {
+                if (captured) (void)hipGraphDestroy(captured);
+                return rc;
+            }
+            if (ce != hipSuccess) return set_error(TFG_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&E->graph, captured, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(captured);
+            if (ie != hipSuccess) return set_error(TFG_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
+        }
+        for (int k = 0; k < nsteps - 1; k++) TFG_HIP(hipGraphLaunch(E->graph, E->stream));
+        if ((rc = enqueue_step<real>(E, true, nullptr)) != TFG_OK) return rc;
+    } else {
+        if (E->profile && static_cast<int>(E->prof_ev.size()) < 4 * nsteps) {
+            for (size_t k = E->prof_ev.size(); k < static_cast<size_t>(4 * nsteps); k++) {
+                hipEvent_t e;
+                TFG_HIP(hipEventCreate(&e));
+                E->prof_ev.push_back(e);
+            }
+        }
+        for (int k = 0; k < nsteps; k++) {
+            hipEvent_t *ev = E->profile ? &E->prof_ev[4 * k] : (k == nsteps - 1 ? E->ev : nullptr);
+            if ((rc = enqueue_step<real>(E, k == nsteps - 1, ev)) != TFG_OK) return rc;
+        }
+    }
+    TFG_HIP(hipEventRecord(E->ev[5], E->stream));
+    TFG_HIP(hipStreamSynchronize(E->stream));
+    TFG_HIP(hipEventElapsedTime(&E->last_ms, E->ev[4], E->ev[5]));
+    if (!use_graph || nsteps <= 1) {
+        hipEvent_t *last = E->profile ? &E->prof_ev[4 * (nsteps - 1)] : E->ev;
+        for (int k = 0; k < 3; k++) {
+            float ms = 0;
+            TFG_HIP(hipEventElapsedTime(&ms, last[k], last[k + 1]));
+            E->kernel_ms[k] = ms;
+        }
+        if (E->profile) {
+            for (int k = 0; k < 3; k++) E->prof_ms[k] = 0;
+            for (int st = 0; st < nsteps; st++) {
+                hipEvent_t *ev = &E->prof_ev[4 * st];
+                for (int k = 0; k < 3; k++) {
+                    float ms = 0;
+                    TFG_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+                    E->prof_ms[k] += ms;
+                }
+            }
+            E->prof_steps = nsteps;
+        }
+    }
+    int err = 0;
+    TFG_HIP(hipMemcpy(&err, E->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+        TFG_HIP(hipMemset(E->d_err, 0, sizeof(int)));
+        return set_error(TFG_E_TOOFAST, "particle moving too fast (moved two cells or more in one step)");
+    }
+    return TFG_OK;
+}
+
+int ensemble_step(DeviceEnsemble *E, int nsteps, bool use_graph) {
+    if (nsteps < 0) return set_error(TFG_E_INVALIDARG, "negative step count");
+    if (nsteps == 0) return TFG_OK;
+    TFG_HIP(hipSetDevice(E->device));
+    int rc = E->precision == PREC_MIXED ? step_t<float>(E, nsteps, use_graph) : step_t<double>(E, nsteps, use_graph);
+    if (rc != TFG_OK) return rc;
+    for (Engine *e : E->engines) e->time += nsteps;
+    return TFG_OK;
+}
+
+// Force evaluation without the update: binning, nonbonded and bonded passes;
+// buffer B keeps the same state (sorted), forces land in A order.
+template <typename real> static int force_t(DeviceEnsemble *E) {
+    KP<real> P = make_params<real>(E, 1);
+    P.step_ahead = 1;   // the evaluation belongs to the next step (engine_step increments time first)
+    int rc;
+    TFG_HIP(hipEventRecord(E->ev[0], E->stream));
+    if ((rc = run_bin<real>(E, P)) != TFG_OK) return rc;
+    TFG_HIP(hipEventRecord(E->ev[1], E->stream));
+    if ((rc = run_nonbond<real>(E, P)) != TFG_OK) return rc;
+    TFG_HIP(hipEventRecord(E->ev[2], E->stream));
+    if ((rc = run_advance<real>(E, P, 0)) != TFG_OK) return rc;
+    TFG_HIP(hipEventRecord(E->ev[3], E->stream));
+    if ((rc = run_energy<real>(E, P)) != TFG_OK) return rc;
+    TFG_HIP(hipStreamSynchronize(E->stream));
+    for (int k = 0; k < 3; k++) {
+        float ms = 0;
+        TFG_HIP(hipEventElapsedTime(&ms, E->ev[k], E->ev[k + 1]));
+        E->kernel_ms[k] = ms;
+    }
+    return TFG_OK;
+}
+
+int ensemble_force(DeviceEnsemble *E) {
+    TFG_HIP(hipSetDevice(E->device));
+    return E->precision == PREC_MIXED ? force_t<float>(E) : force_t<double>(E);
+}
+
+int ensemble_energies(DeviceEnsemble *E, int s, Energies *out) {
+    if (s < 0 || s >= E->nsys) return set_error(TFG_E_RANGE, "system index out of range");
+    double e[5];
+    TFG_HIP(hipStreamSynchronize(E->stream));
+    TFG_HIP(hipMemcpy(e, E->d_energy + 5 * s, sizeof(e), hipMemcpyDeviceToHost));
+    out->epot_nonbond = e[0];
+    out->virial = e[1];
+    out->ekin = e[2];
+    out->epot_bond = e[3];
+    out->epot_angle = e[4];
+    out->epot = e[0] + e[3] + e[4];
+    return TFG_OK;
+}
+
+int ensemble_species_totals(DeviceEnsemble *E, int s, double *out, int cap) {
+    if (s < 0 || s >= E->nsys) return set_error(TFG_E_RANGE, "system index out of range");
+    double t[kMaxSpecies];
+    TFG_HIP(hipStreamSynchronize(E->stream));
+    TFG_HIP(hipMemcpy(t, E->d_spec_tot + kMaxSpecies * s, sizeof(t), hipMemcpyDeviceToHost));
+    const int n = std::min(cap, E->S);
+    for (int k = 0; k < n; k++) out[k] = t[k];
+    return n;
+}
+
+int ensemble_refresh_tables(DeviceEnsemble *E) {
+    (void)E;
+    return set_error(TFG_E_STATE, "potentials cannot change on the device; call from_device, change, to_device");
+}
+
+int ensemble_set_launch(DeviceEnsemble *E, int blocks, int threads) {
+    if (blocks < 0 || threads < 0) return set_error(TFG_E_INVALIDARG, "negative launch size");
+    if (threads > 0 && (threads % 64 != 0 || threads > 1024))
+        return set_error(TFG_E_INVALIDARG, "threads must be a multiple of 64 up to 1024");
+    if (blocks > 0) E->shape.nonbond_blocks = blocks;
+    if (threads > 0) E->shape.stream_threads = threads;
+    if (E->graph) {
+        (void)hipGraphExecDestroy(E->graph);
+        E->graph = nullptr;
+    }
+    return TFG_OK;
+}
+
+int ensemble_timing(DeviceEnsemble *E, double *ms_out, int cap, const char **names_out) {
+    static const char *names[6] = {"bin", "nonbond", "advance", "bin_avg", "nonbond_avg", "advance_avg"};
+    const int n = std::min(cap, 6);
+    for (int k = 0; k < n; k++) {
+        if (k < 3)
+            ms_out[k] = E->kernel_ms[k];
+        else
+            ms_out[k] = E->prof_steps > 0 ? E->prof_ms[k - 3] / E->prof_steps : 0.0;
+        if (names_out) names_out[k] = names[k];
+    }
+    return n;
+}
+
+int ensemble_set_profile(DeviceEnsemble *E, bool on) {
+    E->profile = on;
+    E->prof_steps = 0;
+    return TFG_OK;
+}
+
+double ensemble_last_ms(DeviceEnsemble *E) { return E->last_ms; }
+int ensemble_precision(DeviceEnsemble *E) { return E->precision; }
+int ensemble_size(DeviceEnsemble *E) { return E->nsys; }
+
+template KP<float> make_params<float>(DeviceEnsemble *, int);
+template KP<double> make_params<double>(DeviceEnsemble *, int);
+
+}  // namespace tfg

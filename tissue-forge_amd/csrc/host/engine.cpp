@@ -88,7 +88,19 @@ int Engine::add_type(const std::string &name, double mass, double radius, int dy
     return static_cast<int>(types.size()) - 1;
 }
 
+static bool same_potential(const Potential &a, const Potential &b) {
+    if (a.kind != b.kind || a.flags != b.flags || a.n != b.n || a.a != b.a || a.b != b.b ||
+        a.r0_plusone != b.r0_plusone || a.dpd_alpha != b.dpd_alpha || a.dpd_gamma != b.dpd_gamma ||
+        a.dpd_sigma != b.dpd_sigma || a.c != b.c)
+        return false;
+    for (int k = 0; k < 4; k++)
+        if (a.alpha[k] != b.alpha[k]) return false;
+    return true;
+}
+
 int Engine::add_potential(const Potential &p) {
+    for (size_t k = 0; k < pots.size(); k++)
+        if (same_potential(*pots[k], p)) return static_cast<int>(k);
     pots.push_back(std::make_shared<Potential>(p));
     return static_cast<int>(pots.size()) - 1;
 }

@@ -112,6 +112,8 @@ class Engine {
     int init(const double origin[3], const double dim[3], const int cells[3],
              double cutoff, int max_types, double dt, int nr_fluxsteps);
     int add_type(const std::string &name, double mass, double radius, int dynamics);
+    // Registers a copy of p; an identical table already registered is reused,
+    // so a potential bound to several type pairs keeps a single table.
     int add_potential(const Potential &p);
     int bind(int pot, int ti, int tj);
     int add_flux(int kind, int ta, int tb, int ia, int ib, double k, double decay,
