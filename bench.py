@@ -7,9 +7,14 @@ with the particles resident in HBM.
 
 Prints one JSON line with particle-steps/s, the FP64 GPU path beside it,
 the force error of both paths against the FP64 CPU oracle, the roofline of
-the nonbonded kernel (HIP events on the module's stream) and the CPU
-baseline (the oracle restatement of the reference CPU engine, timed on a
-bounded number of steps on this host's cores).
+the nonbonded kernel (HIP events on the module's stream around every phase of
+every timed step), the CPU baseline (the oracle restatement of the reference
+CPU engine, timed on a bounded number of steps on this host's cores) and the
+512-replica ensemble, batched in one launch sequence against the same
+replicas stepped one after another.
+
+--calibrate-copy runs only a plain device copy of known size (the reference
+for scaling the FETCH_SIZE / WRITE_SIZE counters of a profiler run).
 """
 import argparse
 import importlib.util
@@ -30,6 +35,20 @@ HBM_PEAK_GBS = 8000.0
 # particle's 16-byte position record is read by the 27 cells of its stencil,
 # plus its own force write (27*16 + 16).
 NONBOND_BYTES_PER_PARTICLE = 448.0
+
+
+def traffic_bytes():
+    """HBM bytes per launch of the nonbonded kernel on this workload, from the
+    committed profiler summary (FETCH_SIZE + WRITE_SIZE, each scaled by the
+    same counter's reading for a plain copy of known size); None without it."""
+    path = os.path.join(ROOT, "profiles", "k_nonbond_morse_lj_2M_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch"):
+                return float(line.split("=")[1])
+    return None
 
 
 def load_package():
@@ -54,22 +73,69 @@ def force_errors(fg, fo):
 
 
 def time_gpu(tf, W, scene, pots, precision, steps, warmup):
+    """Timed region: `steps` whole steps with HIP events around each phase
+    (bin, nonbond, advance) on the module's stream; then the same number of
+    steps replayed from the captured graph, without events."""
     eng, _ = W.build_engine(scene, potentials=pots)
     eng.to_device(precision)
     eng.force()
     forces = eng.particles()["f"]
     eng.step(max(warmup, 1))
     eng.set_profiling(True)
-    eng.step(steps)   # device timing of the timed region (events around each phase)
+    t0 = time.perf_counter()
+    eng.step(steps)
+    wall = time.perf_counter() - t0
     kt = eng.kernel_times()
     eng.set_profiling(False)
     t0 = time.perf_counter()
     eng.step(steps)
-    wall = time.perf_counter() - t0
-    dev_ms = eng.last_step_ms()
+    wall_graph = time.perf_counter() - t0
     eng.from_device()
     eng.close()
-    return dict(wall_s=wall, dev_ms=dev_ms, kernels=kt, forces=forces)
+    return dict(wall_s=wall, wall_graph_s=wall_graph, kernels=kt, forces=forces)
+
+
+def bench_ensemble(tf, W, nrep, steps, warmup):
+    """The replica ensemble advanced as one system list (graph replay of the
+    batched step) and the same replicas as separate engines, one after
+    another."""
+    scenes = W.ensemble(nrep=nrep)
+    total = int(sum(sc["x"].shape[0] for sc in scenes))
+    engines = [W.build_engine(sc)[0] for sc in scenes]
+    ens = tf.Ensemble(engines, precision=tf.PREC_MIXED)
+    ens.step(max(warmup, 1), use_graph=True)
+    t0 = time.perf_counter()
+    ens.step(steps, use_graph=True)
+    batched = time.perf_counter() - t0
+    ens.close(download=False)
+    engines = [W.build_engine(sc)[0] for sc in scenes]
+    for e in engines:
+        e.to_device(tf.PREC_MIXED)
+        e.step(max(warmup, 1))
+    t0 = time.perf_counter()
+    for e in engines:
+        e.step(steps)
+    sequential = time.perf_counter() - t0
+    for e in engines:
+        e.close()
+    return {"replicas": nrep, "particles": total, "steps": steps,
+            "batched_ms_per_step": batched / steps * 1e3, "sequential_ms_per_step": sequential / steps * 1e3,
+            "batched_value": total * steps / batched, "sequential_value": total * steps / sequential,
+            "speedup": sequential / batched}
+
+
+def calibrate_copy(nbytes, reps):
+    """A plain device-to-device copy of nbytes, reps times (torch's copy
+    kernel): reads and writes exactly nbytes each."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copied %d bytes %d times" % (n * 4, reps))
 
 
 def main():
@@ -78,15 +144,23 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=2_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--ens-steps", type=int, default=50)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X, multi-GPU runs are not part "
               "of this build", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy, max(args.steps, 1))
+        return
 
     tf = load_package()
     from tissue_forge_amd import workloads as W
@@ -99,8 +173,8 @@ def main():
     log("workload morse_lj n=%d L=%.3f cells=%s" % (n, scene["dim"][0], scene["cells"]))
 
     mixed = time_gpu(tf, W, scene, pots, tf.PREC_MIXED, args.steps, args.warmup)
-    log("mixed: wall %.3f s for %d steps, device %.3f ms, kernels %s" %
-        (mixed["wall_s"], args.steps, mixed["dev_ms"], mixed["kernels"]))
+    log("mixed: wall %.3f s for %d steps (graph replay %.3f s), kernels %s" %
+        (mixed["wall_s"], args.steps, mixed["wall_graph_s"], mixed["kernels"]))
     fp64 = None
     if not args.no_fp64:
         fp64 = time_gpu(tf, W, scene, pots, tf.PREC_FP64, args.steps, args.warmup)
@@ -128,9 +202,10 @@ def main():
                                "cells %s" % (n, scene["cells"]),
                    "parallelism": "single GPU"},
         "phases_ms": {k: mixed["kernels"][k] for k in ("bin_avg", "nonbond_avg", "advance_avg")},
+        "graph_replay_value": n * args.steps / mixed["wall_graph_s"],
         "roofline": {"bound": "hbm", "kernel": "k_nonbond", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_particle": NONBOND_BYTES_PER_PARTICLE},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(),
+                     "bytes_per_particle": NONBOND_BYTES_PER_PARTICLE, "avg_launch_ms": nb_ms},
     }
     if fp64 is not None:
         v64 = n * args.steps / fp64["wall_s"]
@@ -156,6 +231,9 @@ def main():
                                "sample": "%d full steps of the same %d-particle system (FP64 restatement of the "
                                          "reference CPU engine: cell tasks, Newton-3, sorted sweeps)" %
                                          (args.cpu_steps, n)}
+    if not args.no_ensemble:
+        out["ensemble"] = bench_ensemble(tf, W, args.replicas, args.ens_steps, args.warmup)
+        log("ensemble %s" % out["ensemble"])
     print(json.dumps(out), flush=True)
 
 

@@ -121,14 +121,30 @@ int tfg_engine_addangles(tfg_engine *e, const tfg_potential *p, int n, const int
  *      (source/cuda/tfEngineConfig.cpp:67-172). ---- */
 int tfg_device_count(void);
 int tfg_device_set(int ordinal);
+/* cuda::EngineConfig::clearDevice (source/cuda/tfEngineConfig.cpp:57-65): back to device 0. */
+int tfg_device_clear(void);
 /* Uploads the engine; a second call while on the device is a no-op. */
 int tfg_engine_to_device(tfg_engine *e, int precision);
-/* Downloads x, v, f, cell ids and species, frees the device copy. */
+/* Downloads x, v, f, cell ids and species, frees the device copy; a no-op
+ * when the engine is not on the device (source/mdcore/src/tfEngine.cpp:1178-1179). */
 int tfg_engine_from_device(tfg_engine *e);
+/* cuda::EngineConfig::refresh / refreshPotentials / refreshFluxes
+ * (source/cuda/tfEngineConfig.cpp:126-172): the device copy is rebuilt from
+ * the current state.  Every setter above (types, potentials, fluxes,
+ * particles, flags, bonds, angles, noise) does this by itself when the engine
+ * is on the device, as engine_addpot / engine_addfluxes / space_addpart do
+ * (source/mdcore/src/tfEngine.cpp:1070-1073, 1093-1096, tfSpace.cpp:493-496). */
+int tfg_engine_refresh(tfg_engine *e);
 int tfg_engine_on_device(const tfg_engine *e);
 /* cuda::EngineConfig::setBlocks / setThreads (source/cuda/tfEngineConfig.cpp:123-160):
  * workgroups of the pair kernel, threads of the streaming kernels (0 = keep). */
 int tfg_set_launch(tfg_engine *e, int blocks, int threads);
+/* Launch-shape autotuner (no reference counterpart: the reference takes
+ * setBlocks / setThreads by hand).  Times reps force-only passes per
+ * candidate grid of the pair kernel, then of the streaming kernels, and keeps
+ * the fastest; the chosen workgroup counts are returned.  Only shapes that
+ * cannot change a result bit are tuned.  The state is left as it was. */
+int tfg_autotune(tfg_engine *e, int reps, int *blocks, int *stream_blocks);
 
 /* ---- step: engine_step (source/mdcore/src/tfEngine.cpp:1378-1429), whole step on
  *      the device (binning, nonbonded, bonded, flux, forward Euler). ---- */
@@ -171,6 +187,7 @@ int tfg_ensemble_step(tfg_ensemble *ens, int nsteps, int use_graph);
 int tfg_ensemble_destroy(tfg_ensemble *ens, int download);
 double tfg_ensemble_last_ms(const tfg_ensemble *ens);
 int tfg_ensemble_set_launch(tfg_ensemble *ens, int blocks, int threads);
+int tfg_ensemble_autotune(tfg_ensemble *ens, int reps, int *blocks, int *stream_blocks);
 
 #ifdef __cplusplus
 }

@@ -60,8 +60,12 @@ def test_error_codes(tf):
     with pytest.raises(tf.TfgError) as ei:
         tf.Potential.lennard_jones_12_6(0.01, 3.0, 1.0, 1.0, tol=1e-14)
     assert ei.value.code == E_RANGE
-    with pytest.raises(tf.TfgError):
-        eng.from_device()
+    # engine_fromCUDA on an engine that is not on the device does nothing
+    # (source/mdcore/src/tfEngine.cpp:1178-1179)
+    eng.from_device()
+    assert not eng.on_device
+    eng.refresh()   # likewise nothing to rebuild
+    assert not eng.on_device
 
 
 def test_to_device_fails_loudly_without_gpu(tf):

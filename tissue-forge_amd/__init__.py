@@ -79,6 +79,11 @@ _SIGNATURES = {
     "tfg_engine_addangles": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _i32p]),
     "tfg_device_count": (ctypes.c_int, []),
     "tfg_device_set": (ctypes.c_int, [ctypes.c_int]),
+    "tfg_device_clear": (ctypes.c_int, []),
+    "tfg_engine_refresh": (ctypes.c_int, [_vp]),
+    "tfg_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "tfg_ensemble_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int)]),
     "tfg_engine_to_device": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tfg_engine_from_device": (ctypes.c_int, [_vp]),
     "tfg_engine_on_device": (ctypes.c_int, [_vp]),
@@ -138,6 +143,16 @@ def _f64(a, shape=None):
 
 def device_count():
     return lib().tfg_device_count()
+
+
+def set_device(ordinal):
+    """EngineConfig.setDevice: the device that later uploads use."""
+    _check(lib().tfg_device_set(ordinal))
+
+
+def clear_device():
+    """EngineConfig.clearDevice: back to device 0."""
+    _check(lib().tfg_device_clear())
 
 
 class Potential:
@@ -309,12 +324,23 @@ class Engine:
     def from_device(self):
         _check(lib().tfg_engine_from_device(self._h))
 
+    def refresh(self):
+        """Rebuilds the device copy from the current state (EngineConfig.refresh)."""
+        _check(lib().tfg_engine_refresh(self._h))
+
     @property
     def on_device(self):
         return bool(lib().tfg_engine_on_device(self._h))
 
     def set_launch(self, blocks=0, threads=0):
         _check(lib().tfg_set_launch(self._h, blocks, threads))
+
+    def autotune(self, reps=3):
+        """Picks the fastest result-invariant launch shape; returns
+        (pair-kernel workgroups, streaming-kernel workgroups)."""
+        b, s = ctypes.c_int(), ctypes.c_int()
+        _check(lib().tfg_autotune(self._h, reps, ctypes.byref(b), ctypes.byref(s)))
+        return b.value, s.value
 
     # ---- stepping ----
     def step(self, nsteps=1):
@@ -385,6 +411,11 @@ class Ensemble:
 
     def set_launch(self, blocks=0, threads=0):
         _check(lib().tfg_ensemble_set_launch(self._h, blocks, threads))
+
+    def autotune(self, reps=3):
+        b, s = ctypes.c_int(), ctypes.c_int()
+        _check(lib().tfg_ensemble_autotune(self._h, reps, ctypes.byref(b), ctypes.byref(s)))
+        return b.value, s.value
 
     def close(self, download=True):
         if self._h:

@@ -75,6 +75,17 @@ template <typename real> struct PArrays {
     real *q;                     // pending species flux
 };
 
+// Dynamic LDS layout of the pair kernel (bytes): pair descriptors at 0,
+// table coefficients at off_coef, then `waves` staging regions of `region`
+// bytes from off_waves; inside a region positions at 0 (chunk + pad
+// entries) and ids, velocities, radii and species at their offsets.
+struct NbLayout {
+    int waves, chunk;
+    int off_coef, off_waves, region;
+    int off_id, off_vel, off_rad, off_sp;
+    int total;
+};
+
 template <typename real> struct KP {
     const SysDesc *sys;
     const int *cell_sys;
@@ -99,6 +110,7 @@ template <typename real> struct KP {
     int ncells, np, S, energy;
     int step_ahead;   // 1 when forces are evaluated outside a step (time + 1)
     int coef_lds;     // 1 when every system's pair tables fit the LDS cache
+    NbLayout nb;
 };
 
 __device__ inline float as_real_bits(int v, float) { return __int_as_float(v); }
@@ -108,6 +120,22 @@ __device__ inline int real_bits_as_int(double v) { return static_cast<int>(__dou
 
 __device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ inline double rsq(double x) { return 1.0 / sqrt(x); }
+// Lowest set bit of m; 31 when m == 0 (callers then discard the result).
+// v_ffbl_b32 gives -1 for a zero word, so the high word's index + 32 wraps to
+// 31 when it is empty and the low word's index wins whenever it exists.
+// (The instruction is written out: the ctz builtins are undefined for zero,
+// and the compiler then drops the high word.)
+__device__ inline int ctz64_or_31(unsigned long long m) {
+    unsigned lo, hi;
+    asm("v_ffbl_b32 %0, %1" : "=v"(lo) : "v"(static_cast<unsigned>(m)));
+    asm("v_ffbl_b32 %0, %1" : "=v"(hi) : "v"(static_cast<unsigned>(m >> 32)));
+    hi += 32u;
+    return static_cast<int>(lo < hi ? lo : hi);
+}
+
+__device__ inline float clamp_real(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }
+__device__ inline double clamp_real(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
+
 template <typename real> __device__ inline real eps_of();
 template <> __device__ inline float eps_of<float>() { return 1.1920928955078125e-07f; }
 template <> __device__ inline double eps_of<double>() { return 2.220446049250313e-16; }

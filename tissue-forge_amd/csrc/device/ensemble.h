@@ -37,7 +37,7 @@ struct Buffers {
 
 // Launch shape of the pair kernel and the streaming kernels.
 struct LaunchShape {
-    int nonbond_blocks = 0;   // persistent workgroups of the pair kernel (0 = 8 per CU)
+    int nonbond_blocks = 0;   // workgroups of the pair kernel (0 = as many as are resident)
     int stream_threads = 256;
     int stream_blocks = 0;
 };
@@ -52,7 +52,8 @@ struct DeviceEnsemble {
     int max_sys_blocks = 1;   // energy partial blocks per system
     bool has_dpd = false, has_flux = false, has_radius = false;
     int coef_lds_bytes = 0;   // table bytes cached in LDS by the pair kernel (0: read from global)
-    int nonbond_lds = 0;      // dynamic LDS of the pair kernel (pair descriptors + tables)
+    NbLayout nb{};            // LDS layout of the pair kernel
+    int nonbond_resident = 0; // pair-kernel workgroups resident on the device at once
     LaunchShape shape;
     Buffers buf;
     hipStream_t stream = nullptr;
@@ -97,8 +98,10 @@ struct DeviceEnsemble {
         return want < 1 ? 1 : (want < cap ? want : cap);
     }
     int nonbond_grid() const {
-        const int cap = shape.nonbond_blocks > 0 ? shape.nonbond_blocks : cus * 8;
-        return ncells < cap ? ncells : cap;
+        const int cap = shape.nonbond_blocks > 0 ? shape.nonbond_blocks : nonbond_resident;
+        const int want = (ncells + nb.waves - 1) / (nb.waves > 0 ? nb.waves : 1);
+        const int g = want < cap ? want : cap;
+        return g < 1 ? 1 : g;
     }
 };
 
@@ -112,7 +115,8 @@ template <typename real> int run_bin(DeviceEnsemble *E, const KP<real> &P);
 // Counts particles per cell from buffer B (after an upload).
 int run_count(DeviceEnsemble *E);
 template <typename real> int run_nonbond(DeviceEnsemble *E, const KP<real> &P);
-template <typename real> int nonbond_lds_bytes(int max_types, int max_coef, bool coef_lds);
+// Chooses the LDS layout and the resident grid of the pair kernel.
+template <typename real> int nonbond_setup(DeviceEnsemble *E, int max_types, int max_coef);
 template <typename real> int run_advance(DeviceEnsemble *E, const KP<real> &P, int update);
 template <typename real> int run_energy(DeviceEnsemble *E, const KP<real> &P);
 

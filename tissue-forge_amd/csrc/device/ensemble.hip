@@ -64,6 +64,7 @@ template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy) {
     P.energy = energy;
     P.step_ahead = 0;
     P.coef_lds = E->coef_lds_bytes > 0 ? 1 : 0;
+    P.nb = E->nb;
     return P;
 }
 
@@ -282,12 +283,10 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     int max_np = 1;
     for (const SysDesc &d : sys) max_np = std::max(max_np, d.np);
     E->max_sys_blocks = (max_np + energy_tile() - 1) / energy_tile();
-    // Pair tables of a system are cached in LDS when the largest fits 32 KiB.
-    const size_t cbytes = static_cast<size_t>(max_coef) * sizeof(real);
-    E->coef_lds_bytes = (cbytes > 0 && cbytes <= 32768) ? static_cast<int>(cbytes) : 0;
     int max_nt = 1;
     for (const SysDesc &d : sys) max_nt = std::max(max_nt, d.ntypes);
-    E->nonbond_lds = nonbond_lds_bytes<real>(max_nt, max_coef, E->coef_lds_bytes > 0);
+    int rc = nonbond_setup<real>(E, max_nt, max_coef);
+    if (rc != TFG_OK) return rc;
 
     // Particles in id order (slot == global id) into buffer B, and the bonded
     // incidence lists (CSR by global particle id).
@@ -348,7 +347,6 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     }
 
     Buffers &B = E->buf;
-    int rc;
 #define ALLOC(ptr, T, n)                                            \
     do {                                                            \
         T *p_ = nullptr;                                            \
@@ -640,9 +638,67 @@ int ensemble_species_totals(DeviceEnsemble *E, int s, double *out, int cap) {
     return n;
 }
 
-int ensemble_refresh_tables(DeviceEnsemble *E) {
-    (void)E;
-    return set_error(TFG_E_STATE, "potentials cannot change on the device; call from_device, change, to_device");
+// Launch-shape autotuner: times force-only passes (state-preserving: the
+// update is off and buffer B receives A unchanged) over candidate grids of
+// the pair kernel, then of the streaming kernels, and keeps the fastest.
+// Only result-invariant parameters are tuned: every cell's sums have the
+// same order whatever workgroup computes them, so tuning never changes a
+// result bit.
+int ensemble_autotune(DeviceEnsemble *E, int reps, int *blocks_out, int *stream_blocks_out) {
+    if (reps < 1) reps = 1;
+    TFG_HIP(hipSetDevice(E->device));
+    auto run = [&](int which) -> double {
+        double best = 1e30;
+        for (int r = 0; r < reps; r++) {
+            const int rc = E->precision == PREC_MIXED ? force_t<float>(E) : force_t<double>(E);
+            if (rc != TFG_OK) return -1.0;
+            const double t = which == 1 ? E->kernel_ms[1] : E->kernel_ms[0] + E->kernel_ms[2];
+            best = std::min(best, t);
+        }
+        return best;
+    };
+    const int res = std::max(1, E->nonbond_resident);
+    const int most = std::max(1, (E->ncells + E->nb.waves - 1) / E->nb.waves);
+    std::vector<int> cand;
+    for (int c : {res / 2, res, 2 * res, 3 * res, 4 * res, most})
+        if (c >= 1 && c <= most && std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+    int best_b = E->shape.nonbond_blocks;
+    double best_t = 1e30;
+    for (int c : cand) {
+        E->shape.nonbond_blocks = c;
+        const double t = run(1);
+        if (t < 0) return TFG_E_DEVICE;
+        if (t < best_t) {
+            best_t = t;
+            best_b = c;
+        }
+    }
+    E->shape.nonbond_blocks = best_b;
+    int best_s = E->shape.stream_blocks;
+    best_t = 1e30;
+    for (int c : {E->cus * 4, E->cus * 8, E->cus * 16, E->cus * 32}) {
+        E->shape.stream_blocks = c;
+        const double t = run(0);
+        if (t < 0) return TFG_E_DEVICE;
+        if (t < best_t) {
+            best_t = t;
+            best_s = c;
+        }
+    }
+    E->shape.stream_blocks = best_s;
+    if (E->graph) {
+        (void)hipGraphExecDestroy(E->graph);
+        E->graph = nullptr;
+    }
+    if (blocks_out) *blocks_out = best_b;
+    if (stream_blocks_out) *stream_blocks_out = best_s;
+    return TFG_OK;
+}
+
+int ensemble_get_launch(DeviceEnsemble *E, int *blocks, int *threads) {
+    *blocks = E->shape.nonbond_blocks;
+    *threads = E->shape.stream_threads;
+    return TFG_OK;
 }
 
 int ensemble_set_launch(DeviceEnsemble *E, int blocks, int threads) {

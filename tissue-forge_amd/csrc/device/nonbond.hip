@@ -7,19 +7,19 @@
 // tf_flux_eval.h:48-103), and of the CUDA sort/self/pair kernels
 // (tfRunner_cuda.cu:2469-2845).
 //
-// Work unit: one wave per cell of i particles.  A persistent workgroup walks
-// a contiguous range of cells (neighbouring cells share most of their
-// stencil, so their loads hit the same cache lines).  The particles of the
-// stencil cells are staged in LDS, shifted into the frame of the i cell (the
-// shift is added in FP64 and rounded once).  Lanes are grouped as
-// (i particle, slice of the staged list).  A round gives every lane up to 64
-// candidates: the distance test over them is fully unrolled and sets one bit
-// per in-range candidate in a 64-bit register mask; the lane then evaluates
-// only the set bits.  The table evaluation thus runs on dense lanes instead
-// of the ~15% of lanes whose candidate is inside the cutoff, and no LDS is
-// spent on pair lists.  Slices are combined with cross-lane reductions in a
-// fixed order: no atomics, results do not depend on scheduling.  The pair
-// descriptors and table coefficients of the current system live in LDS.
+// Work unit: one wave per cell of i particles.  A workgroup of kNbWaves waves
+// owns a contiguous range of cells (workgroups of one XCD own neighbouring
+// ranges, so their stencils meet in that XCD's L2); its waves take the cells
+// of the range in turn and share the pair descriptors and table
+// coefficients of the current system in LDS.  Each wave stages the particles
+// of its stencil in its own LDS region, shifted into the frame of the i cell
+// (the shift is added in FP64 and rounded once).  Lanes are grouped as
+// (i particle, slice of the staged list), with as many slices as fit the 64
+// lanes.  Per round a lane tests a contiguous block of up to 128 candidates
+// into a register bit mask (the test is unrolled with immediate LDS offsets;
+// blocks of different slices start on different LDS banks), then evaluates
+// the set bits, two per iteration.  Slices are combined in a fixed order
+// through LDS: no atomics, results do not depend on scheduling.
 #include <algorithm>
 
 #include "ensemble.h"
@@ -27,163 +27,222 @@
 
 namespace tfg {
 
-constexpr int kChunk = 256;   // staged neighbour particles per round
-constexpr int kRound = 64;    // candidates per lane per round (bits of the mask)
+constexpr int kNbWaves = 4;              // waves per workgroup
+constexpr int kNbThreads = kNbWaves * kWave;
+constexpr int kPad = 16;                 // staged entries read past the end by the unrolled test
+constexpr int kCoefLdsMax = 16384;       // table bytes cached in LDS
+constexpr int kWaveLdsBudget = 40960;    // staging bytes of a workgroup
 
-// Pair descriptor in LDS, two vectors per type pair:
-//   table: q0 = (a, b, alpha0, alpha1), q1 = (alpha2, r0+1, coff bits, meta bits)
+// Pair descriptor in LDS, kPairVecs vectors per type pair:
+//   table: q0 = (a, b, alpha0, alpha1), q1 = (alpha2, n, table bits, meta bits), q2 = (r0+1, 0, 0, 0)
 //   DPD:   q0 = (a, b, alpha, gamma),    q1 = (sigma, 1/b, 0, meta bits)
-// meta = n | kind << 16 | shifted << 20 | scaled << 21 | present << 24.
+// An absent pair has a = 0, b = -1, so that no distance is in range.  The
+// table bits are the LDS byte offset of the table (from the start of the
+// dynamic LDS) when the tables are cached, else its index in the global
+// coefficient array.  meta = kind << 16 | shifted << 20 | scaled << 21.
+// Staged particles carry their type as the byte offset of its descriptor
+// column (type * kPairBytes), the i particle its row (type * ntypes * kPairBytes).
+constexpr int kPairVecs = 4;
 constexpr int kMetaDpd = 1 << 16;
 constexpr int kMetaShifted = 1 << 20;
 constexpr int kMetaScaled = 1 << 21;
-constexpr int kMetaPresent = 1 << 24;
 
-template <typename real>
-__host__ __device__ inline int pair_lds_bytes(int ntypes) {
-    return ntypes * ntypes * 2 * static_cast<int>(sizeof(typename V<real>::v4));
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS>
-__global__ __launch_bounds__(64) void k_nonbond(KP<real> P) {
+static inline int align16(int b) { return (b + 15) & ~15; }
+
+// Dynamic LDS layout: pair descriptors, table coefficients, then one
+// staging region per wave (positions, ids, velocities, radii, species).
+template <typename real>
+NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, bool radius, int max_chunk) {
+    using v4 = typename V<real>::v4;
+    const bool ids = dpd || flux;
+    NbLayout L{};
+    L.waves = kNbWaves;
+    L.off_coef = align16(max_nt * max_nt * kPairVecs * static_cast<int>(sizeof(v4)));
+    L.off_waves = L.off_coef + align16(coef_lds_count * static_cast<int>(sizeof(real)));
+    auto region = [&](int chunk, NbLayout *o) {
+        int b = align16((chunk + kPad) * static_cast<int>(sizeof(v4)));
+        o->off_id = b;
+        if (ids) b += align16(chunk * 4);
+        o->off_vel = b;
+        if (dpd) b += align16(chunk * static_cast<int>(sizeof(v4)));
+        o->off_rad = b;
+        if (radius) b += align16(chunk * static_cast<int>(sizeof(real)));
+        o->off_sp = b;
+        if (flux) b += align16(chunk * S * static_cast<int>(sizeof(real)));
+        // the slice reduction reuses the region: 64 lanes x (6 + S) values
+        return std::max(b, kWave * (6 + S) * static_cast<int>(sizeof(real)));
+    };
+    int chunk = max_chunk;
+    while (chunk > 64 && kNbWaves * region(chunk, &L) > kWaveLdsBudget) chunk /= 2;
+    L.chunk = chunk;
+    L.region = region(chunk, &L);
+    L.total = L.off_waves + kNbWaves * L.region;
+    return L;
+}
+
+template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY>
+__global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
     using v4 = typename V<real>::v4;
     constexpr bool kIds = DPD || FLUX;   // pair terms that need the partner's id
+    constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
     extern __shared__ __align__(16) unsigned char s_dyn[];
-    __shared__ v4 s_pos[kChunk];
-    __shared__ int s_slot[kIds ? kChunk : 1];
-    __shared__ v4 s_vel[DPD ? kChunk : 1];
-    __shared__ real s_rad[RADIUS ? kChunk : 1];
-    __shared__ real s_sp[FLUX ? kChunk * kMaxSpecies : 1];
     __shared__ FluxDesc<real> s_flux[FLUX ? kMaxFluxEntries : 1];
     __shared__ int2 s_fmap[FLUX ? kMaxTypes * kMaxTypes : 1];
 
-    const int lane = threadIdx.x;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform in the wave
+    const int lane = threadIdx.x & (kWave - 1);
     const int S = P.S;
-    const int per = (P.ncells + gridDim.x - 1) / gridDim.x;
-    const int c_begin = blockIdx.x * per;
-    const int c_end = min(P.ncells, c_begin + per);
-    int cur_sys = -1;
+    const int chunk = P.nb.chunk;
     v4 *s_pair = reinterpret_cast<v4 *>(s_dyn);
-    real *s_coef = nullptr;
+    real *s_coef = reinterpret_cast<real *>(s_dyn + P.nb.off_coef);
+    (void)s_coef;
+    unsigned char *region = s_dyn + P.nb.off_waves + wid * P.nb.region;
+    v4 *s_pos = reinterpret_cast<v4 *>(region);
+    int *s_id = reinterpret_cast<int *>(region + P.nb.off_id);
+    v4 *s_vel = reinterpret_cast<v4 *>(region + P.nb.off_vel);
+    real *s_rad = reinterpret_cast<real *>(region + P.nb.off_rad);
+    real *s_sp = reinterpret_cast<real *>(region + P.nb.off_sp);
+    real *s_red = reinterpret_cast<real *>(region);
 
-    for (int c = c_begin; c < c_end; c++) {
-        const int i0 = P.start[c];
-        const int ni = P.start[c + 1] - i0;
-        if (ni == 0) continue;
-        const int sysid = P.cell_sys[c];
+    // Workgroups are dispatched to the XCDs in turn; give the workgroups of
+    // one XCD consecutive cell ranges.
+    const int nblk = gridDim.x;
+    const int blk = (nblk % 8 == 0) ? (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
+    const int per_blk = (P.ncells + nblk - 1) / nblk;
+    const int c_begin = min(P.ncells, blk * per_blk);
+    const int c_end = min(P.ncells, c_begin + per_blk);
+
+    for (int seg = c_begin; seg < c_end;) {
+        const int sysid = P.cell_sys[seg];
         const SysDesc &sd = P.sys[sysid];
+        const int seg_end = min(c_end, sd.cell_off + sd.ncells);
         const int nt = sd.ntypes;
-        s_coef = reinterpret_cast<real *>(s_dyn + pair_lds_bytes<real>(nt));
-        if (sysid != cur_sys) {
-            __syncthreads();
-            for (int t = lane; t < nt * nt; t += kWave) {
-                const int pi = P.potmap[sd.pot_off + t];
-                v4 q0, q1;
-                q0.x = q0.y = q0.z = q0.w = real(0);
-                q1 = q0;
-                int meta = 0;
-                if (pi >= 0) {
-                    const PotDesc<real> pd = P.pots[pi];
-                    meta = kMetaPresent | (pd.kind == KIND_DPD ? kMetaDpd : 0) |
-                           ((pd.flags & POT_SHIFTED) ? kMetaShifted : 0) | ((pd.flags & POT_SCALED) ? kMetaScaled : 0) |
-                           (pd.n & 0xffff);
-                    q0.x = pd.a;
-                    q0.y = pd.b;
-                    if (pd.kind == KIND_DPD) {
-                        q0.z = pd.dalpha;
-                        q0.w = pd.dgamma;
-                        q1.x = pd.dsigma;
-                        q1.y = real(1) / pd.b;
-                    } else {
-                        q0.z = pd.al0;
-                        q0.w = pd.al1;
-                        q1.x = pd.al2;
-                        q1.y = pd.r0p1;
-                        q1.z = as_real_bits(CLDS ? pd.coff - sd.coef_begin : pd.coff, real(0));
-                    }
+        // Pair descriptors (and tables) of this system.
+        __syncthreads();
+        for (int t = threadIdx.x; t < nt * nt; t += kNbThreads) {
+            const int pi = P.potmap[sd.pot_off + t];
+            v4 q0, q1, q2, q3;
+            q0.x = q0.y = q0.z = q0.w = real(0);
+            q1 = q2 = q3 = q0;
+            q0.y = real(-1);   // absent: nothing in range
+            int meta = 0;
+            if (pi >= 0) {
+                const PotDesc<real> pd = P.pots[pi];
+                meta = (pd.kind == KIND_DPD ? kMetaDpd : 0) | ((pd.flags & POT_SHIFTED) ? kMetaShifted : 0) |
+                       ((pd.flags & POT_SCALED) ? kMetaScaled : 0);
+                q0.x = pd.a;
+                q0.y = pd.b;
+                if (pd.kind == KIND_DPD) {
+                    q0.z = pd.dalpha;
+                    q0.w = pd.dgamma;
+                    q1.x = pd.dsigma;
+                    q1.y = real(1) / pd.b;
+                } else {
+                    q0.z = pd.al0;
+                    q0.w = pd.al1;
+                    q1.x = pd.al2;
+                    q1.y = real(pd.n);
+                    q1.z = as_real_bits(CLDS ? P.nb.off_coef + (pd.coff - sd.coef_begin) * static_cast<int>(sizeof(real))
+                                             : pd.coff,
+                                        real(0));
+                    q2.x = pd.r0p1;
                 }
-                q1.w = as_real_bits(meta, real(0));
-                s_pair[2 * t] = q0;
-                s_pair[2 * t + 1] = q1;
-                if (FLUX) s_fmap[t] = make_int2(P.fluxmap[2 * (sd.flux_off + t)], P.fluxmap[2 * (sd.flux_off + t) + 1]);
             }
-            if (FLUX)
-                for (int t = lane; t < sd.nflux; t += kWave) s_flux[t] = P.flux[sd.flux_first + t];
-            if (CLDS)
-                for (int t = lane; t < sd.coef_count; t += kWave) s_coef[t] = P.coef[sd.coef_begin + t];
-            __syncthreads();
-            cur_sys = sysid;
+            q1.w = as_real_bits(meta, real(0));
+            s_pair[kPairVecs * t] = q0;
+            s_pair[kPairVecs * t + 1] = q1;
+            s_pair[kPairVecs * t + 2] = q2;
+            s_pair[kPairVecs * t + 3] = q3;
+            if (FLUX) s_fmap[t] = make_int2(P.fluxmap[2 * (sd.flux_off + t)], P.fluxmap[2 * (sd.flux_off + t) + 1]);
         }
+        if (FLUX)
+            for (int t = threadIdx.x; t < sd.nflux; t += kNbThreads) s_flux[t] = P.flux[sd.flux_first + t];
+        if (CLDS)
+            for (int t = threadIdx.x; t < sd.coef_count; t += kNbThreads) s_coef[t] = P.coef[sd.coef_begin + t];
+        __syncthreads();
+
         const real cut2 = real(sd.cutoff2);
-        const real icut = real(1.0 / sd.cutoff);
-        const real wnorm = real(sd.w_self);   // 1 / (pi rc^3)
-        int loc[3];
-        cell_loc(c - sd.cell_off, sd, loc);
+        // number density W(r, rc) = (1 - 1.5 x^2 + 0.75 x^3) / (pi rc^3), x = r / rc
+        // (w_cubic_spline, tf_smoothing_kernel.h:42-59), as w0 + r^2 (w2 + r w3)
+        const real w0 = real(sd.w_self);
+        const real w2 = real(-1.5 * sd.w_self / sd.cutoff2);
+        const real w3 = real(0.75 * sd.w_self / (sd.cutoff2 * sd.cutoff));
         const long long tstep = sd.time0 + *P.step + P.step_ahead;
         const real dpd_delta = real(sd.inv_sqrt_dt);
+        const real eps2 = eps_of<real>() * eps_of<real>();
 
-        for (int ib = 0; ib < ni; ib += kWave) {
-            const int nib = min(kWave, ni - ib);
-            int slices = 1;
-            while (slices < kWave && nib * slices * 2 <= kWave) slices *= 2;
-            const int il = lane / slices;
-            const int sl = lane % slices;
-            const bool active = il < nib;
-            const int ii = ib + (active ? il : 0);   // index of the i particle in its cell
-            const int si = i0 + ii;
-            const v4 pi4 = P.A.pos[si];
-            const int4 mi = P.A.meta[si];
-            const int ti = mi.x & 0xffff;
-            v4 vi4;
-            if (DPD) vi4 = P.A.vel[si];
-            const int id_i = mi.y - sd.p_off;
-            real spi[kMaxSpecies];
-            if (FLUX)
-                for (int k = 0; k < S; k++) spi[k] = P.A.sp[(size_t)k * P.np + si];
+        for (int c = seg + wid; c < seg_end; c += kNbWaves) {
+            const int i0 = P.start[c];
+            const int ni = P.start[c + 1] - i0;
+            if (ni == 0) continue;
+            int loc[3];
+            cell_loc(c - sd.cell_off, sd, loc);
 
-            real fx = 0, fy = 0, fz = 0, nd = 0, ep = 0, vir = 0;
-            real qacc[kMaxSpecies];
-            if (FLUX)
-                for (int k = 0; k < S; k++) qacc[k] = 0;
+            for (int ib = 0; ib < ni; ib += kWave) {
+                const int nib = min(kWave, ni - ib);
+                const int slices = kWave / nib;
+                const int il = lane / slices;
+                const int sl = lane - il * slices;
+                const bool active = il < nib;
+                const int ii = ib + (active ? il : 0);   // index of the i particle in its cell
+                const int si = i0 + ii;
+                const v4 pi4 = P.A.pos[si];
+                const int4 mi = P.A.meta[si];
+                const int ti = mi.x & 0xffff;
+                const int ti_row = ti * nt * kPairBytes;
+                v4 vi4;
+                if (DPD) vi4 = P.A.vel[si];
+                const int id_i = mi.y - sd.p_off;
+                real spi[kMaxSpecies];
+                if (FLUX)
+                    for (int k = 0; k < S; k++) spi[k] = P.A.sp[(size_t)k * P.np + si];
 
-            // Staged range of the i cell in the current chunk: entry k holds
-            // particle (k - own_off) of that cell for own_lo <= k < own_hi.
-            int own_lo = 0, own_hi = 0, own_off = 0;
+                real fx = 0, fy = 0, fz = 0, nd = 0, ep = 0, vir = 0;
+                real qacc[kMaxSpecies];
+                if (FLUX)
+                    for (int k = 0; k < S; k++) qacc[k] = 0;
+                int self_k = -1;   // staged index of the i particle itself in the current chunk
 
-            // One candidate inside the cutoff (index k of the staged list).
-            auto eval_pair = [&](int k) {
-                if (k >= own_lo && k < own_hi && k - own_off == ii) return;   // the particle itself
-                const v4 pj = s_pos[k];
-                const real dx = pi4.x - pj.x;
-                const real dy = pi4.y - pj.y;
-                const real dz = pi4.z - pj.z;
-                const real r2 = dx * dx + dy * dy + dz * dz;
-                const real ir = rsq(r2);
-                const real r_raw = r2 * ir;
-                // number density: w_cubic_spline (tf_smoothing_kernel.h:42-59); r <= rc here
-                {
-                    const real x = r_raw * icut;
-                    const real x2 = x * x;
-                    nd += (real(1) - real(1.5) * x2 + real(0.75) * x2 * x) * wnorm;
-                }
-                const int tj = real_bits_as_int(pj.w) & 0xffff;
-                const int tp = ti * nt + tj;
-                int id_j = 0;
-                if (kIds) id_j = P.A.meta[s_slot[k]].y - sd.p_off;
-                if (FLUX) {
-                    const int2 fm = s_fmap[tp];
-                    if (fm.y > 0) {
-                        const real r = r_raw;
+                // One candidate k of the staged list; ok = false makes it a no-op
+                // (k is then 0, a staged entry, so every read stays in range).
+                auto eval_pair = [&](int k, bool ok) {
+                    const v4 pj = s_pos[k];
+                    const real dx = pi4.x - pj.x;
+                    const real dy = pi4.y - pj.y;
+                    const real dz = pi4.z - pj.z;
+                    const real r2 = dx * dx + dy * dy + dz * dz;
+                    // below eps the inverse distance is 1/eps (r itself only
+                    // enters where it is clamped to the table's a, or through
+                    // W(r) ~ W(0))
+                    const real ir = rsq(fmax(r2, eps2));
+                    const real r_raw = r2 * ir;
+                    nd += ok ? w0 + r2 * (w2 + r_raw * w3) : real(0);
+                    const int tpb = ti_row + real_bits_as_int(pj.w);   // descriptor byte offset
+                    const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + tpb);
+                    const v4 q0 = qd[0];
+                    const v4 q1 = qd[1];
+                    const int meta = real_bits_as_int(q1.w);
+                    int id_j = 0;
+                    if (kIds) id_j = s_id[k] - sd.p_off;
+                    if (FLUX && ok) {
+                        const int tj = real_bits_as_int(pj.w) / kPairBytes;
+                        const int2 fm = s_fmap[ti * nt + tj];
                         for (int f = 0; f < fm.y; f++) {
                             const FluxDesc<real> &fd = s_flux[fm.x + f];
-                            if (r > fd.cutoff) continue;
-                            real term = real(1) - r / fd.cutoff;
+                            if (r_raw > fd.cutoff) continue;
+                            real term = real(1) - r_raw / fd.cutoff;
                             term = term * term;
                             // Role of this particle in the entry (tf_flux_eval.h:64-67):
                             // the type_a side; between equal types the lower id.
                             const bool i_is_a = (ti == fd.type_a) && (ti != tj || id_i < id_j);
-                            const real sa = i_is_a ? spi[fd.index_a] : s_sp[k * kMaxSpecies + fd.index_a];
-                            const real sb = i_is_a ? s_sp[k * kMaxSpecies + fd.index_b] : spi[fd.index_b];
+                            const real sa = i_is_a ? spi[fd.index_a] : s_sp[k * S + fd.index_a];
+                            const real sb = i_is_a ? s_sp[k * S + fd.index_b] : spi[fd.index_b];
                             real qv;
                             if (fd.kind == FLUX_FICK) {
                                 qv = fd.coef * (sa - sb);
@@ -201,249 +260,309 @@ __global__ __launch_bounds__(64) void k_nonbond(KP<real> P) {
                                 qacc[fd.index_b] += qv - fd.half_decay * sb;
                         }
                     }
-                }
-                const v4 q1 = s_pair[2 * tp + 1];
-                const int meta = real_bits_as_int(q1.w);
-                if (!(meta & kMetaPresent)) return;
-                const v4 q0 = s_pair[2 * tp];
-                const bool tiny = r_raw < eps_of<real>();
-                const real iro = tiny ? real(1) / eps_of<real>() : ir;
-                if (DPD && (meta & kMetaDpd)) {
-                    const real ro = tiny ? eps_of<real>() : r_raw;
+                    if (DPD && (meta & kMetaDpd)) {
+                        const real ro = r2 < eps2 ? eps_of<real>() : r_raw;
+                        real r = r_raw;
+                        if (RADIUS && (meta & kMetaShifted)) r = r - (pi4.w + s_rad[k]);
+                        const bool in = ok && r <= q0.y;
+                        r = r >= q0.x ? r : q0.x;
+                        const real ex = dx * ir, ey = dy * ir, ez = dz * ir;
+                        const v4 vj = s_vel[k];
+                        const real ev = ex * (vi4.x - vj.x) + ey * (vi4.y - vj.y) + ez * (vi4.z - vj.z);
+                        const real om = r < real(0) ? real(1) : (real(1) - r * q1.y);
+                        real xi = real(1);
+                        if (sd.dpd_noise == DPD_NOISE_PHILOX)
+                            xi = dpd_xi<real>(static_cast<uint32_t>(id_i), static_cast<uint32_t>(id_j),
+                                              static_cast<uint64_t>(tstep), sd.seed_lo, sd.seed_hi);
+                        real f = q0.z * om - q0.w * om * om * ev + q1.x * om * dpd_delta * xi;
+                        f = in ? f : real(0);
+                        fx += f * ex;
+                        fy += f * ey;
+                        fz += f * ez;
+                        if (ENERGY) vir += f * ro;
+                        return;
+                    }
+                    // Table potential (potential_eval_ex).
                     real r = r_raw;
-                    if (RADIUS && (meta & kMetaShifted)) r = r - (pi4.w + s_rad[k]);
-                    if (r > q0.y) return;
-                    r = r >= q0.x ? r : q0.x;
-                    const real ex = dx * iro, ey = dy * iro, ez = dz * iro;
-                    const v4 vj = s_vel[k];
-                    const real ev = ex * (vi4.x - vj.x) + ey * (vi4.y - vj.y) + ez * (vi4.z - vj.z);
-                    const real om = r < real(0) ? real(1) : (real(1) - r * q1.y);
-                    real xi = real(1);
-                    if (sd.dpd_noise == DPD_NOISE_PHILOX)
-                        xi = dpd_xi<real>(static_cast<uint32_t>(id_i), static_cast<uint32_t>(id_j),
-                                          static_cast<uint64_t>(tstep), sd.seed_lo, sd.seed_hi);
-                    const real f = q0.z * om - q0.w * om * om * ev + q1.x * om * dpd_delta * xi;
-                    fx += f * ex;
-                    fy += f * ey;
-                    fz += f * ez;
-                    vir += f * ro;
-                    return;
-                }
-                // Table potential (potential_eval_ex).
-                real r = r_raw;
-                if (RADIUS) {
-                    if (meta & kMetaScaled)
-                        r = r / (pi4.w + s_rad[k]);
-                    else if (meta & kMetaShifted)
-                        r = r - (pi4.w + s_rad[k]) + q1.y;
-                }
-                r = r < q0.x ? q0.x : r;
-                if (r > q0.y) return;
-                const real t = q0.z + r * (q0.w + r * q1.x);
-                const int n = meta & 0xffff;
-                int ind = static_cast<int>(t > real(0) ? t : real(0));
-                ind = ind < 1 ? 1 : (ind > n ? n : ind);
-                const int co = real_bits_as_int(q1.z) + ind * 8;
-                const real *cc = CLDS ? s_coef + co : P.coef + co;
-                const real x = (r - cc[0]) * cc[1];
-                real ee = cc[2] * x + cc[3];
-                real eff = cc[2];
-                eff = eff * x + ee; ee = ee * x + cc[4];
-                eff = eff * x + ee; ee = ee * x + cc[5];
-                eff = eff * x + ee; ee = ee * x + cc[6];
-                eff = eff * x + ee; ee = ee * x + cc[7];
-                const real f = eff * cc[1] * iro;
-                fx -= f * dx;
-                fy -= f * dy;
-                fz -= f * dz;
-                ep += ee;
-                vir -= f * r2;
-            };
-
-            // Rounds of up to 64 candidates per lane: each lane tests a
-            // contiguous block of the staged list (immediate LDS offsets) into
-            // a bit mask, then evaluates the set bits.
-            auto process = [&](int count) {
-                for (int rb = 0; rb < count; rb += kRound * slices) {
-                    const int rcount = min(count - rb, kRound * slices);
-                    const int per = (rcount + slices - 1) / slices;
-                    const int kb = rb + sl * per;
-                    const int my = min(per, rb + rcount - kb);
-                    unsigned long long mask = 0;
-                    if (active) {
-                        const v4 *sp4 = s_pos + kb;
-                        if (my == kRound) {
-#pragma unroll
-                            for (int t = 0; t < kRound; t++) {
-                                const v4 pj = sp4[t];
-                                const real dx = pi4.x - pj.x;
-                                const real dy = pi4.y - pj.y;
-                                const real dz = pi4.z - pj.z;
-                                const real r2 = dx * dx + dy * dy + dz * dz;
-                                mask |= static_cast<unsigned long long>(r2 <= cut2) << t;
-                            }
-                        } else {
-                            for (int t = 0; t < my; t++) {
-                                const v4 pj = sp4[t];
-                                const real dx = pi4.x - pj.x;
-                                const real dy = pi4.y - pj.y;
-                                const real dz = pi4.z - pj.z;
-                                const real r2 = dx * dx + dy * dy + dz * dz;
-                                mask |= static_cast<unsigned long long>(r2 <= cut2) << t;
-                            }
-                        }
+                    if (RADIUS) {
+                        if (meta & kMetaScaled)
+                            r = r / (pi4.w + s_rad[k]);
+                        else if (meta & kMetaShifted)
+                            r = r - (pi4.w + s_rad[k]) + qd[2].x;
                     }
-                    while (__any(mask != 0ull)) {
-                        if (mask) {
-                            const int t = __builtin_ctzll(mask);
-                            mask &= mask - 1ull;
-                            eval_pair(kb + t);
-                        }
-                    }
-                }
-            };
-
-            int nstaged = 0;
-            const int sz = sd.span_z;
-            for (int col = 0; col < sd.nstencil; col++) {
-                const int ox = P.stencil[2 * (sd.stencil_off + col)];
-                const int oy = P.stencil[2 * (sd.stencil_off + col) + 1];
-                int n0 = loc[0] + ox;
-                n0 += n0 < 0 ? sd.cdim[0] : 0;
-                n0 -= n0 >= sd.cdim[0] ? sd.cdim[0] : 0;
-                int n1 = loc[1] + oy;
-                n1 += n1 < 0 ? sd.cdim[1] : 0;
-                n1 -= n1 >= sd.cdim[1] ? sd.cdim[1] : 0;
-                const int colbase = sd.cell_off + (n0 * sd.cdim[1] + n1) * sd.cdim[2];
-                const double shx = ox * sd.h[0], shy = oy * sd.h[1];
-                const int zlo = loc[2] - sz, zhi = loc[2] + sz;
-                // The column's cells are contiguous in memory unless the z range
-                // wraps around the periodic boundary; then it is two runs.
-                const bool wrap_lo = zlo < 0, wrap_hi = zhi >= sd.cdim[2];
-                const int nruns = (wrap_lo || wrap_hi) ? 2 : 1;
-                for (int run = 0; run < nruns; run++) {
-                    int zf, nzc, oz0;
-                    if (!wrap_lo && !wrap_hi) {
-                        zf = zlo; nzc = 2 * sz + 1; oz0 = -sz;
-                    } else if (wrap_lo) {
-                        if (run == 0) { zf = zlo + sd.cdim[2]; nzc = -zlo; oz0 = -sz; }
-                        else { zf = 0; nzc = zhi + 1; oz0 = -loc[2]; }
+                    r = fmax(r, q0.x);
+                    const bool in = ok && r <= q0.y;
+                    // interval in [1, n] (the clamp is exact: t is rounded down after it)
+                    const int ind = static_cast<int>(clamp_real(q0.z + r * (q0.w + r * q1.x), real(1), q1.y));
+                    const int tb = real_bits_as_int(q1.z);
+                    v4 c0, c1;
+                    if (CLDS) {
+                        const v4 *c4 = reinterpret_cast<const v4 *>(s_dyn + tb + ind * 8 * static_cast<int>(sizeof(real)));
+                        c0 = c4[0];
+                        c1 = c4[1];
                     } else {
-                        if (run == 0) { zf = zlo; nzc = sd.cdim[2] - zlo; oz0 = -sz; }
-                        else { zf = 0; nzc = zhi - sd.cdim[2] + 1; oz0 = sd.cdim[2] - loc[2]; }
+                        const v4 *c4 = reinterpret_cast<const v4 *>(P.coef + tb + ind * 8);
+                        c0 = c4[0];
+                        c1 = c4[1];
                     }
-                    int bnd[2 * kMaxSpanZ + 2];
+                    const real x = (r - c0.x) * c0.y;
+                    real ee = c0.z * x + c0.w;
+                    real eff = c0.z;
+                    eff = eff * x + ee; ee = ee * x + c1.x;
+                    eff = eff * x + ee; ee = ee * x + c1.y;
+                    eff = eff * x + ee; ee = ee * x + c1.z;
+                    eff = eff * x + ee; ee = ee * x + c1.w;
+                    const real f = in ? eff * c0.y * ir : real(0);
+                    fx -= f * dx;
+                    fy -= f * dy;
+                    fz -= f * dz;
+                    if (ENERGY) {
+                        ep += in ? ee : real(0);
+                        vir -= f * r2;
+                    }
+                };
+
+                // Candidates [0, count) of the staged list.
+                auto process = [&](int count) {
+                    const int cap = slices > 1 ? 127 : 128;
+                    for (int rb = 0; rb < count; rb += cap * slices) {
+                        const int rcount = min(count - rb, cap * slices);
+                        int per = (rcount + slices - 1) / slices;
+                        if (slices > 1) per |= 1;   // odd stride: slices start on different banks
+                        const int kb = rb + sl * per;
+                        const int my = active ? max(0, min(per, rb + rcount - kb)) : 0;
+                        const v4 *sp4 = s_pos + kb;
+                        unsigned long long m0 = 0, m1 = 0;
 #pragma unroll
-                    for (int m = 0; m < 2 * kMaxSpanZ + 2; m++)
-                        bnd[m] = m <= nzc ? P.start[colbase + zf + m] : 0x7fffffff;
-                    int ja = bnd[0];
-                    const int jend = P.start[colbase + zf + nzc];
-                    while (ja < jend) {
-                        const int take = min(jend - ja, kChunk - nstaged);
-                        for (int t = lane; t < take; t += kWave) {
-                            const int sj = ja + t;
-                            int oz = oz0;
+                        for (int blk = 0; blk < 8; blk++) {
+                            if (blk * 16 < my) {
+                                unsigned bits = 0;
 #pragma unroll
-                            for (int m = 1; m < 2 * kMaxSpanZ + 1; m++) oz += (m < nzc && sj >= bnd[m]) ? 1 : 0;
-                            v4 p = P.A.pos[sj];
-                            const int info = P.A.meta[sj].x;
-                            if (RADIUS) s_rad[nstaged + t] = p.w;
-                            p.x = real(static_cast<double>(p.x) + shx);
-                            p.y = real(static_cast<double>(p.y) + shy);
-                            p.z = real(static_cast<double>(p.z) + oz * sd.h[2]);
-                            p.w = as_real_bits(info, real(0));
-                            s_pos[nstaged + t] = p;
-                            if (kIds) s_slot[nstaged + t] = sj;
-                            if (DPD) s_vel[nstaged + t] = P.A.vel[sj];
-                            if (FLUX)
-                                for (int k = 0; k < S; k++)
-                                    s_sp[(nstaged + t) * kMaxSpecies + k] = P.A.sp[(size_t)k * P.np + sj];
+                                for (int t = 0; t < 16; t++) {
+                                    const v4 pj = sp4[blk * 16 + t];
+                                    const real dx = pi4.x - pj.x;
+                                    const real dy = pi4.y - pj.y;
+                                    const real dz = pi4.z - pj.z;
+                                    const real r2 = dx * dx + dy * dy + dz * dz;
+                                    bits |= static_cast<unsigned>(r2 <= cut2) << t;
+                                }
+                                const unsigned long long w = static_cast<unsigned long long>(bits) << ((blk & 3) * 16);
+                                if (blk < 4)
+                                    m0 |= w;
+                                else
+                                    m1 |= w;
+                            }
                         }
-                        // staged range of the i cell's own particles
-                        const int olo = max(i0, ja), ohi = min(i0 + ni, ja + take);
-                        if (olo < ohi) {
-                            own_lo = nstaged + olo - ja;
-                            own_hi = nstaged + ohi - ja;
-                            own_off = nstaged - ja + i0;
+                        // drop the entries past this lane's block and the particle itself
+                        m0 &= my >= 64 ? ~0ull : ((1ull << my) - 1ull);
+                        m1 &= my >= 128 ? ~0ull : (my <= 64 ? 0ull : ((1ull << (my - 64)) - 1ull));
+                        const int st = self_k - kb;
+                        if (st >= 0 && st < 64) m0 &= ~(1ull << st);
+                        if (st >= 64 && st < 128) m1 &= ~(1ull << (st - 64));
+                        // Evaluate the set bits, two per iteration.  m0 is refilled
+                        // from m1 when it runs empty; an empty pick gives ok = false.
+                        int base = kb;
+                        const int klim = max(count - 1, 0);
+                        while (__any((m0 | m1) != 0ull)) {
+                            const bool refill = m0 == 0ull;
+                            m0 = refill ? m1 : m0;
+                            m1 = refill ? 0ull : m1;
+                            base += refill ? 64 : 0;
+                            int k2[2];
+                            bool ok2[2];
+#pragma unroll
+                            for (int u = 0; u < 2; u++) {
+                                ok2[u] = m0 != 0ull;
+                                k2[u] = min(base + ctz64_or_31(m0), klim);
+                                m0 &= m0 - 1ull;
+                            }
+                            eval_pair(k2[0], ok2[0]);
+                            eval_pair(k2[1], ok2[1]);
                         }
-                        nstaged += take;
-                        ja += take;
-                        if (nstaged == kChunk) {
-                            __syncthreads();
-                            process(nstaged);
-                            __syncthreads();
-                            nstaged = 0;
-                            own_lo = own_hi = 0;
+                    }
+                };
+
+                int nstaged = 0;
+                const int sz = sd.span_z;
+                for (int col = 0; col < sd.nstencil; col++) {
+                    const int ox = P.stencil[2 * (sd.stencil_off + col)];
+                    const int oy = P.stencil[2 * (sd.stencil_off + col) + 1];
+                    int n0 = loc[0] + ox;
+                    n0 += n0 < 0 ? sd.cdim[0] : 0;
+                    n0 -= n0 >= sd.cdim[0] ? sd.cdim[0] : 0;
+                    int n1 = loc[1] + oy;
+                    n1 += n1 < 0 ? sd.cdim[1] : 0;
+                    n1 -= n1 >= sd.cdim[1] ? sd.cdim[1] : 0;
+                    const int colbase = sd.cell_off + (n0 * sd.cdim[1] + n1) * sd.cdim[2];
+                    const double shx = ox * sd.h[0], shy = oy * sd.h[1];
+                    const int zlo = loc[2] - sz, zhi = loc[2] + sz;
+                    // The column's cells are contiguous in memory unless the z range
+                    // wraps around the periodic boundary; then it is two runs.
+                    const bool wrap_lo = zlo < 0, wrap_hi = zhi >= sd.cdim[2];
+                    const int nruns = (wrap_lo || wrap_hi) ? 2 : 1;
+                    for (int run = 0; run < nruns; run++) {
+                        int zf, nzc, oz0;
+                        if (!wrap_lo && !wrap_hi) {
+                            zf = zlo; nzc = 2 * sz + 1; oz0 = -sz;
+                        } else if (wrap_lo) {
+                            if (run == 0) { zf = zlo + sd.cdim[2]; nzc = -zlo; oz0 = -sz; }
+                            else { zf = 0; nzc = zhi + 1; oz0 = -loc[2]; }
+                        } else {
+                            if (run == 0) { zf = zlo; nzc = sd.cdim[2] - zlo; oz0 = -sz; }
+                            else { zf = 0; nzc = zhi - sd.cdim[2] + 1; oz0 = sd.cdim[2] - loc[2]; }
+                        }
+                        // first slot of each cell of the run (cells 1 .. nzc-1 set oz)
+                        int bnd[2 * kMaxSpanZ + 1];
+#pragma unroll
+                        for (int m = 0; m < 2 * kMaxSpanZ + 1; m++)
+                            bnd[m] = m <= nzc ? P.start[colbase + zf + m] : 0x7fffffff;
+                        int ja = bnd[0];
+                        const int jend = P.start[colbase + zf + nzc];
+                        while (ja < jend) {
+                            const int take = min(jend - ja, chunk - nstaged);
+                            for (int t = lane; t < take; t += kWave) {
+                                const int sj = ja + t;
+                                int oz = oz0;
+#pragma unroll
+                                for (int m = 1; m < 2 * kMaxSpanZ + 1; m++) {
+                                    if (m >= nzc) break;
+                                    oz += sj >= bnd[m] ? 1 : 0;
+                                }
+                                v4 p = P.A.pos[sj];
+                                const int2 mj = *reinterpret_cast<const int2 *>(&P.A.meta[sj]);
+                                if (RADIUS) s_rad[nstaged + t] = p.w;
+                                p.x = real(static_cast<double>(p.x) + shx);
+                                p.y = real(static_cast<double>(p.y) + shy);
+                                p.z = real(static_cast<double>(p.z) + oz * sd.h[2]);
+                                p.w = as_real_bits((mj.x & 0xffff) * kPairBytes, real(0));
+                                s_pos[nstaged + t] = p;
+                                if (kIds) s_id[nstaged + t] = mj.y;
+                                if (DPD) s_vel[nstaged + t] = P.A.vel[sj];
+                                if (FLUX)
+                                    for (int k = 0; k < S; k++)
+                                        s_sp[(nstaged + t) * S + k] = P.A.sp[(size_t)k * P.np + sj];
+                            }
+                            if (si >= ja && si < ja + take) self_k = nstaged + si - ja;
+                            nstaged += take;
+                            ja += take;
+                            if (nstaged == chunk) {
+                                wave_sync();
+                                process(nstaged);
+                                wave_sync();
+                                nstaged = 0;
+                                self_k = -1;
+                            }
                         }
                     }
                 }
-            }
-            if (nstaged > 0) {
-                __syncthreads();
-                process(nstaged);
-                __syncthreads();
-            }
-            // Combine slices (lanes il*slices .. il*slices+slices-1).
-            for (int off = slices >> 1; off > 0; off >>= 1) {
-                fx += __shfl_xor(fx, off);
-                fy += __shfl_xor(fy, off);
-                fz += __shfl_xor(fz, off);
-                nd += __shfl_xor(nd, off);
-                ep += __shfl_xor(ep, off);
-                vir += __shfl_xor(vir, off);
-                if (FLUX)
-                    for (int k = 0; k < S; k++) qacc[k] += __shfl_xor(qacc[k], off);
-            }
-            if (active && sl == 0) {
-                v4 fo;
-                fo.x = fx;
-                fo.y = fy;
-                fo.z = fz;
-                fo.w = nd + real(sd.w_self);
-                P.force[si] = fo;
-                if (FLUX)
-                    for (int k = 0; k < S; k++) P.A.q[(size_t)k * P.np + si] += qacc[k];
-                if (P.energy) {
-                    P.e_nb[si] = 0.5 * static_cast<double>(ep);
-                    P.e_vir[si] = 0.5 * static_cast<double>(vir);
+                if (nstaged > 0) {
+                    wave_sync();
+                    process(nstaged);
+                    wave_sync();
+                }
+                // Combine the slices of each i particle in slice order.
+                if (slices > 1) {
+                    s_red[0 * kWave + lane] = fx;
+                    s_red[1 * kWave + lane] = fy;
+                    s_red[2 * kWave + lane] = fz;
+                    s_red[3 * kWave + lane] = nd;
+                    s_red[4 * kWave + lane] = ep;
+                    s_red[5 * kWave + lane] = vir;
+                    if (FLUX)
+                        for (int k = 0; k < S; k++) s_red[(6 + k) * kWave + lane] = qacc[k];
+                    wave_sync();
+                    if (active && sl == 0) {
+                        fx = fy = fz = nd = ep = vir = real(0);
+                        if (FLUX)
+                            for (int k = 0; k < S; k++) qacc[k] = real(0);
+                        for (int s = 0; s < slices; s++) {
+                            fx += s_red[0 * kWave + lane + s];
+                            fy += s_red[1 * kWave + lane + s];
+                            fz += s_red[2 * kWave + lane + s];
+                            nd += s_red[3 * kWave + lane + s];
+                            ep += s_red[4 * kWave + lane + s];
+                            vir += s_red[5 * kWave + lane + s];
+                            if (FLUX)
+                                for (int k = 0; k < S; k++) qacc[k] += s_red[(6 + k) * kWave + lane + s];
+                        }
+                    }
+                    wave_sync();
+                }
+                if (active && sl == 0) {
+                    v4 fo;
+                    fo.x = fx;
+                    fo.y = fy;
+                    fo.z = fz;
+                    fo.w = nd + real(sd.w_self);
+                    P.force[si] = fo;
+                    if (FLUX)
+                        for (int k = 0; k < S; k++) P.A.q[(size_t)k * P.np + si] += qacc[k];
+                    if (ENERGY) {
+                        P.e_nb[si] = 0.5 * static_cast<double>(ep);
+                        P.e_vir[si] = 0.5 * static_cast<double>(vir);
+                    }
                 }
             }
         }
+        seg = seg_end;
     }
-}
-
-template <typename real> int nonbond_lds_bytes(int max_types, int max_coef, bool coef_lds) {
-    return pair_lds_bytes<real>(max_types) + (coef_lds ? max_coef * static_cast<int>(sizeof(real)) : 0);
 }
 
 template <typename real, bool DPD, bool FLUX, bool RADIUS>
-static void launch_nonbond_t(DeviceEnsemble *E, const KP<real> &P) {
-    if (P.coef_lds)
-        hipLaunchKernelGGL((k_nonbond<real, DPD, FLUX, RADIUS, true>), dim3(E->nonbond_grid()), dim3(64),
-                           E->nonbond_lds, E->stream, P);
-    else
-        hipLaunchKernelGGL((k_nonbond<real, DPD, FLUX, RADIUS, false>), dim3(E->nonbond_grid()), dim3(64),
-                           E->nonbond_lds, E->stream, P);
+static const void *nonbond_fn(bool clds, bool energy) {
+    if (clds)
+        return energy ? reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, RADIUS, true, true>)
+                      : reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, RADIUS, true, false>);
+    return energy ? reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, RADIUS, false, true>)
+                  : reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, RADIUS, false, false>);
 }
 
-template <typename real> int run_nonbond(DeviceEnsemble *E, const KP<real> &P) {
-    const int key = (E->has_dpd ? 4 : 0) | (E->has_flux ? 2 : 0) | (E->has_radius ? 1 : 0);
+template <typename real> static const void *pick_nonbond(int key, bool clds, bool energy) {
     switch (key) {
-        case 0: launch_nonbond_t<real, false, false, false>(E, P); break;
-        case 1: launch_nonbond_t<real, false, false, true>(E, P); break;
-        case 2: launch_nonbond_t<real, false, true, false>(E, P); break;
-        case 3: launch_nonbond_t<real, false, true, true>(E, P); break;
-        case 4: launch_nonbond_t<real, true, false, false>(E, P); break;
-        case 5: launch_nonbond_t<real, true, false, true>(E, P); break;
-        case 6: launch_nonbond_t<real, true, true, false>(E, P); break;
-        default: launch_nonbond_t<real, true, true, true>(E, P); break;
+        case 0: return nonbond_fn<real, false, false, false>(clds, energy);
+        case 1: return nonbond_fn<real, false, false, true>(clds, energy);
+        case 2: return nonbond_fn<real, false, true, false>(clds, energy);
+        case 3: return nonbond_fn<real, false, true, true>(clds, energy);
+        case 4: return nonbond_fn<real, true, false, false>(clds, energy);
+        case 5: return nonbond_fn<real, true, false, true>(clds, energy);
+        case 6: return nonbond_fn<real, true, true, false>(clds, energy);
+        default: return nonbond_fn<real, true, true, true>(clds, energy);
     }
+}
+
+static int variant_key(const DeviceEnsemble *E) {
+    return (E->has_dpd ? 4 : 0) | (E->has_flux ? 2 : 0) | (E->has_radius ? 1 : 0);
+}
+
+template <typename real> int nonbond_setup(DeviceEnsemble *E, int max_nt, int max_coef) {
+    const size_t cbytes = static_cast<size_t>(max_coef) * sizeof(real);
+    E->coef_lds_bytes = (cbytes > 0 && cbytes <= static_cast<size_t>(kCoefLdsMax)) ? static_cast<int>(cbytes) : 0;
+    const int max_chunk = 512;
+    E->nb = nb_layout<real>(max_nt, E->coef_lds_bytes > 0 ? max_coef : 0, E->S, E->has_dpd, E->has_flux,
+                            E->has_radius, max_chunk);
+    // resident workgroups of the variant that runs every step (no energies)
+    const void *fn = pick_nonbond<real>(variant_key(E), E->coef_lds_bytes > 0, false);
+    int per_cu = 0;
+    TFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNbThreads, E->nb.total));
+    if (per_cu < 1) return set_error(TFG_E_DEVICE, "pair kernel does not fit a compute unit");
+    E->nonbond_resident = per_cu * E->cus;
+    return TFG_OK;
+}
+
+int nonbond_waves() { return kNbWaves; }
+
+template <typename real> int run_nonbond(DeviceEnsemble *E, const KP<real> &P) {
+    const void *fn = pick_nonbond<real>(variant_key(E), P.coef_lds != 0, P.energy != 0);
+    void *args[] = {const_cast<KP<real> *>(&P)};
+    const hipError_t e =
+        hipLaunchKernel(fn, dim3(E->nonbond_grid()), dim3(kNbThreads), args, E->nb.total, E->stream);
+    if (e != hipSuccess) return set_error(TFG_E_DEVICE, std::string("nonbond: ") + hipGetErrorString(e));
     return launch_error("nonbond");
 }
 
 template int run_nonbond<float>(DeviceEnsemble *, const KP<float> &);
 template int run_nonbond<double>(DeviceEnsemble *, const KP<double> &);
-template int nonbond_lds_bytes<float>(int, int, bool);
-template int nonbond_lds_bytes<double>(int, int, bool);
+template int nonbond_setup<float>(DeviceEnsemble *, int, int);
+template int nonbond_setup<double>(DeviceEnsemble *, int, int);
 
 }  // namespace tfg

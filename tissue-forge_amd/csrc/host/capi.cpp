@@ -42,6 +42,35 @@ using namespace tfg;
         return set_error(TFG_E_FAIL, ex.what());                      \
     }
 
+// Applies a host-side change to an engine that may be on the device: the
+// device copy is downloaded and released, the change is made on the host
+// and the engine is uploaded again with the same precision and launch
+// shape.  This is the role of engine_cuda_refresh_pots / _fluxes /
+// _particles (source/mdcore/src/tfEngine.cpp:1070-1073, 1093-1096,
+// source/mdcore/src/tfSpace.cpp:493-496) and cuda::EngineConfig::refresh
+// (source/cuda/tfEngineConfig.cpp:163-172).  The engine is back on the
+// device even when the change itself fails.
+template <typename F> static int with_refresh(tfg_engine *h, F &&change) {
+    if (!h->e.device) return change();
+    DeviceEnsemble *d = h->e.device;
+    if (ensemble_size(d) != 1)
+        return set_error(TFG_E_STATE, "engine belongs to an ensemble; destroy the ensemble before changing it");
+    const int prec = ensemble_precision(d);
+    int blocks = 0, threads = 0;
+    ensemble_get_launch(d, &blocks, &threads);
+    int rc = ensemble_destroy(d, true);
+    if (rc != TFG_OK) return rc;
+    const int rc_change = change();
+    std::string msg = rc_change != TFG_OK ? std::string(last_error()) : std::string();
+    Engine *ep = &h->e;
+    DeviceEnsemble *nd = nullptr;
+    rc = ensemble_create(&nd, &ep, 1, prec);
+    if (rc != TFG_OK) return rc;
+    ensemble_set_launch(nd, blocks, threads);
+    if (rc_change != TFG_OK) return set_error(rc_change, msg);
+    return TFG_OK;
+}
+
 extern "C" {
 
 const char *tfg_last_error(void) { return last_error(); }
@@ -75,6 +104,11 @@ int tfg_engine_destroy(tfg_engine *h) {
     return TFG_OK;
 }
 
+int tfg_engine_refresh(tfg_engine *h) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    TFG_GUARD(return with_refresh(h, [] { return TFG_OK; }));
+}
+
 int tfg_engine_geometry(const tfg_engine *h, int cdim[3], double hh[3], int span[3]) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
     for (int k = 0; k < 3; k++) {
@@ -87,10 +121,11 @@ int tfg_engine_geometry(const tfg_engine *h, int cdim[3], double hh[3], int span
 
 int tfg_engine_set_max_distance_fraction(tfg_engine *h, double fraction) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
-    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");
     if (!(fraction > 0)) return set_error(TFG_E_INVALIDARG, "fraction must be positive");
-    h->e.max_dist_fraction = fraction;
-    return TFG_OK;
+    TFG_GUARD(return with_refresh(h, [&] {
+        h->e.max_dist_fraction = fraction;
+        return TFG_OK;
+    }));
 }
 
 int tfg_engine_set_nspecies(tfg_engine *h, int nspecies) {
@@ -104,19 +139,19 @@ int tfg_engine_set_nspecies(tfg_engine *h, int nspecies) {
 
 int tfg_engine_set_dpd_noise(tfg_engine *h, int mode, uint64_t seed) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
-    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");
     if (mode != DPD_NOISE_UNIT && mode != DPD_NOISE_PHILOX) return set_error(TFG_E_INVALIDARG, "unknown DPD noise mode");
-    h->e.dpd_noise = mode;
-    h->e.dpd_seed = seed;
-    return TFG_OK;
+    TFG_GUARD(return with_refresh(h, [&] {
+        h->e.dpd_noise = mode;
+        h->e.dpd_seed = seed;
+        return TFG_OK;
+    }));
 }
 
 int64_t tfg_engine_time(const tfg_engine *h) { return h ? h->e.time : -1; }
 
 int tfg_engine_addtype(tfg_engine *h, const char *name, double mass, double radius, int dynamics) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
-    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");
-    TFG_GUARD(return h->e.add_type(name ? name : "", mass, radius, dynamics));
+    TFG_GUARD(return with_refresh(h, [&] { return h->e.add_type(name ? name : "", mass, radius, dynamics); }));
 }
 
 static int finish_potential(tfg_potential **out, tfg_potential *p, int rc) {
@@ -223,35 +258,33 @@ int tfg_potential_eval(const tfg_potential *h, double r, double ri, double rj, d
 
 int tfg_engine_addpot(tfg_engine *h, const tfg_potential *p, int ti, int tj) {
     if (!h || !p) return set_error(TFG_E_INVALIDARG, "null argument");
-    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device; potentials cannot change there");
-    TFG_GUARD({
+    TFG_GUARD(return with_refresh(h, [&] {
         const int id = h->e.add_potential(p->p);
         return h->e.bind(id, ti, tj);
-    });
+    }));
 }
 
 int tfg_engine_addflux(tfg_engine *h, int kind, int ta, int tb, int sa, int sb, double k, double decay,
                        double target, double cutoff) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
-    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");
-    TFG_GUARD(return h->e.add_flux(kind, ta, tb, sa, sb, k, decay, target, cutoff));
+    TFG_GUARD(return with_refresh(h, [&] { return h->e.add_flux(kind, ta, tb, sa, sb, k, decay, target, cutoff); }));
 }
 
 int tfg_engine_addparts(tfg_engine *h, int n, const int *types, const double *x, const double *v,
                         const double *species, int32_t *ids_out) {
     if (!h || (n > 0 && (!types || !x))) return set_error(TFG_E_INVALIDARG, "null argument");
-    TFG_GUARD(return h->e.add_particles(n, types, x, v, species, ids_out));
+    TFG_GUARD(return with_refresh(h, [&] { return h->e.add_particles(n, types, x, v, species, ids_out); }));
 }
 
 int tfg_engine_set_particle_flags(tfg_engine *h, int n, const int32_t *ids, const uint32_t *flags) {
     if (!h || (n > 0 && (!ids || !flags))) return set_error(TFG_E_INVALIDARG, "null argument");
-    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");
-    for (int k = 0; k < n; k++) {
+    for (int k = 0; k < n; k++)
         if (ids[k] < 0 || ids[k] >= static_cast<int>(h->e.parts.size()))
             return set_error(TFG_E_RANGE, "particle id out of range");
-        h->e.parts[ids[k]].flags = flags[k] & 0xffffu;
-    }
-    return TFG_OK;
+    TFG_GUARD(return with_refresh(h, [&] {
+        for (int k = 0; k < n; k++) h->e.parts[ids[k]].flags = flags[k] & 0xffffu;
+        return TFG_OK;
+    }));
 }
 
 int tfg_engine_nparts(const tfg_engine *h) { return h ? static_cast<int>(h->e.parts.size()) : -1; }
@@ -262,18 +295,17 @@ static int pot_registry_id(tfg_engine *h, const tfg_potential *p) {
 
 int tfg_engine_addbonds(tfg_engine *h, const tfg_potential *p, int n, const int32_t *ij) {
     if (!h || !p || (n > 0 && !ij)) return set_error(TFG_E_INVALIDARG, "null argument");
-    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");
-    TFG_GUARD(return h->e.add_bonds(pot_registry_id(h, p), n, ij));
+    TFG_GUARD(return with_refresh(h, [&] { return h->e.add_bonds(pot_registry_id(h, p), n, ij); }));
 }
 
 int tfg_engine_addangles(tfg_engine *h, const tfg_potential *p, int n, const int32_t *ijk) {
     if (!h || !p || (n > 0 && !ijk)) return set_error(TFG_E_INVALIDARG, "null argument");
-    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");
-    TFG_GUARD(return h->e.add_angles(pot_registry_id(h, p), n, ijk));
+    TFG_GUARD(return with_refresh(h, [&] { return h->e.add_angles(pot_registry_id(h, p), n, ijk); }));
 }
 
 int tfg_device_count(void) { return device_count(); }
 int tfg_device_set(int ordinal) { return device_select(ordinal); }
+int tfg_device_clear(void) { return device_count() > 0 ? device_select(0) : TFG_OK; }
 
 int tfg_engine_to_device(tfg_engine *h, int precision) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
@@ -285,7 +317,7 @@ int tfg_engine_to_device(tfg_engine *h, int precision) {
 
 int tfg_engine_from_device(tfg_engine *h) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
-    if (!h->e.device) return set_error(TFG_E_STATE, "engine is not on the device");
+    if (!h->e.device) return TFG_OK;   // tfEngine.cpp:1178-1179: not on the device, nothing to do
     if (ensemble_size(h->e.device) != 1)
         return set_error(TFG_E_STATE, "engine belongs to an ensemble; use tfg_ensemble_destroy");
     TFG_GUARD(return ensemble_destroy(h->e.device, true));
@@ -297,6 +329,13 @@ int tfg_set_launch(tfg_engine *h, int blocks, int threads) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
     if (!h->e.device) return set_error(TFG_E_STATE, "engine is not on the device");
     return ensemble_set_launch(h->e.device, blocks, threads);
+}
+
+int tfg_autotune(tfg_engine *h, int reps, int *blocks, int *stream_blocks) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    if (!h->e.device) return set_error(TFG_E_STATE, "engine is not on the device");
+    if (ensemble_size(h->e.device) != 1) return set_error(TFG_E_STATE, "engine belongs to an ensemble");
+    TFG_GUARD(return ensemble_autotune(h->e.device, reps, blocks, stream_blocks));
 }
 
 int tfg_engine_step(tfg_engine *h, int nsteps) {
@@ -444,6 +483,11 @@ double tfg_ensemble_last_ms(const tfg_ensemble *ens) {
 int tfg_ensemble_set_launch(tfg_ensemble *ens, int blocks, int threads) {
     if (!ens || !ens->dev) return set_error(TFG_E_INVALIDARG, "null ensemble");
     return ensemble_set_launch(ens->dev, blocks, threads);
+}
+
+int tfg_ensemble_autotune(tfg_ensemble *ens, int reps, int *blocks, int *stream_blocks) {
+    if (!ens || !ens->dev) return set_error(TFG_E_INVALIDARG, "null ensemble");
+    TFG_GUARD(return ensemble_autotune(ens->dev, reps, blocks, stream_blocks));
 }
 
 }  // extern "C"

@@ -1,4 +1,4 @@
-// Interface between the host engine and the device module (device.hip).
+// Interface between the host engine and the device module (csrc/device).
 // Everything here is host-callable C++; the kernels stay private to the
 // device module.
 #pragma once
@@ -30,9 +30,11 @@ int ensemble_force(DeviceEnsemble *ens);
 int ensemble_energies(DeviceEnsemble *ens, int sys, Energies *out);
 // FP64 per-species totals of one system after the last step; returns count.
 int ensemble_species_totals(DeviceEnsemble *ens, int sys, double *out, int cap);
-// Re-uploads potentials / fluxes of every system (engine_addpot while on the device).
-int ensemble_refresh_tables(DeviceEnsemble *ens);
 int ensemble_set_launch(DeviceEnsemble *ens, int blocks, int threads);
+int ensemble_get_launch(DeviceEnsemble *ens, int *blocks, int *threads);
+// Times candidate grids of the pair kernel and of the streaming kernels on
+// force-only passes and keeps the fastest (results do not depend on them).
+int ensemble_autotune(DeviceEnsemble *ens, int reps, int *blocks, int *stream_blocks);
 // Per-kernel timing of the last ensemble_step / ensemble_force call.
 int ensemble_timing(DeviceEnsemble *ens, double *ms_out, int cap, const char **names_out);
 // Milliseconds of device time for the last nsteps of ensemble_step.
