@@ -80,6 +80,7 @@ def time_gpu(tf, W, scene, pots, precision, steps, warmup):
     eng.to_device(precision)
     eng.force()
     forces = eng.particles()["f"]
+    xq = eng.positions()   # the input positions as the path holds them
     eng.step(max(warmup, 1))
     eng.set_profiling(True)
     t0 = time.perf_counter()
@@ -92,7 +93,7 @@ def time_gpu(tf, W, scene, pots, precision, steps, warmup):
     wall_graph = time.perf_counter() - t0
     eng.from_device()
     eng.close()
-    return dict(wall_s=wall, wall_graph_s=wall_graph, kernels=kt, forces=forces)
+    return dict(wall_s=wall, wall_graph_s=wall_graph, kernels=kt, forces=forces, positions=xq)
 
 
 def bench_ensemble(tf, W, nrep, steps, warmup):
@@ -219,7 +220,21 @@ def main():
         orc = O.Oracle(scene, pots)
         orc.force(threads)
         fo = orc.particles()["f"]
-        out["force_error_vs_fp64_oracle"] = {"mixed": force_errors(mixed["forces"], fo)}
+        # The mixed path holds positions in FP32 (cell-centred frame), so its
+        # input is the seeded configuration rounded to FP32.  Its arithmetic
+        # error is measured against the oracle on exactly those positions;
+        # the error against the unrounded inputs, and what that rounding alone
+        # does to the FP64 oracle, are reported beside it.
+        sq = dict(scene)
+        sq["x"] = mixed["positions"]
+        orq = O.Oracle(sq, pots)
+        orq.force(threads)
+        fq = orq.particles()["f"]
+        out["force_error_vs_fp64_oracle"] = {
+            "mixed": force_errors(mixed["forces"], fq),
+            "mixed_vs_unrounded_inputs": force_errors(mixed["forces"], fo),
+            "fp32_input_rounding_alone": force_errors(fq, fo),
+        }
         if fp64 is not None:
             out["force_error_vs_fp64_oracle"]["fp64_gpu"] = force_errors(fp64["forces"], fo)
         log("oracle setup+force %.1f s" % (time.perf_counter() - t0))

@@ -25,6 +25,9 @@ constexpr int kMaxTypes = 8;          // type pairs cached in LDS by the pair ke
 constexpr int kMaxSpecies = 8;        // TF_SIMD_SIZE
 constexpr int kMaxFluxEntries = 64;   // flux entries per system
 constexpr int kMaxSpanZ = 3;          // cells of cutoff along z per direction
+// FP32 path: pairs within this relative band of the cutoff (or of a table end
+// below it) are decided in FP64 by the pair kernel.
+constexpr double kGuard = 2e-6;
 
 // Per-system constants.
 struct SysDesc {
@@ -38,6 +41,7 @@ struct SysDesc {
     int nflux;
     int coef_begin, coef_count;   // this system's pair-table coefficients
     int dpd_noise;
+    int binside;   // some bound potential's range ends below the cutoff or depends on radii
     uint32_t seed_lo, seed_hi;
     long long time0;   // engine->time before the first device step
     double h[3];
@@ -52,6 +56,7 @@ template <typename real> struct PotDesc {
     int n, coff;
     real a, b, r0p1, al0, al1, al2;
     real dalpha, dgamma, dsigma;
+    double bd;   // b in FP64 (the FP32 path's cutoff guard)
 };
 
 struct TypeDesc {
@@ -78,11 +83,12 @@ template <typename real> struct PArrays {
 // Dynamic LDS layout of the pair kernel (bytes): pair descriptors at 0,
 // table coefficients at off_coef, then `waves` staging regions of `region`
 // bytes from off_waves; inside a region positions at 0 (chunk + pad
-// entries) and ids, velocities, radii and species at their offsets.
+// entries) and ids, slots (FP32 cutoff guard), velocities, radii and
+// species at their offsets.
 struct NbLayout {
     int waves, chunk;
     int off_coef, off_waves, region;
-    int off_id, off_vel, off_rad, off_sp;
+    int off_id, off_slot, off_vel, off_rad, off_sp;
     int total;
 };
 

@@ -218,6 +218,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
             pd.coff = static_cast<int>(coef.size());
             pd.a = real(p.a);
             pd.b = real(p.b);
+            pd.bd = p.b;
             pd.r0p1 = real(p.r0_plusone);
             pd.al0 = real(p.alpha[0]);
             pd.al1 = real(p.alpha[1]);
@@ -239,6 +240,8 @@ template <typename real> static int upload(DeviceEnsemble *E) {
                     const Potential &p = *e.pots[pid];
                     if (p.kind == KIND_DPD) E->has_dpd = true;
                     if (p.flags & (POT_SHIFTED | POT_SCALED)) E->has_radius = true;
+                    if (p.b < e.geom.cutoff * (1.0 - 4.0 * kGuard) || (p.flags & (POT_SHIFTED | POT_SCALED)))
+                        d.binside = 1;
                 }
                 potmap.push_back(pid >= 0 ? add_pot(pid) : -1);
             }

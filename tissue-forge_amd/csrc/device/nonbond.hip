@@ -20,7 +20,17 @@
 // blocks of different slices start on different LDS banks), then evaluates
 // the set bits, two per iteration.  Slices are combined in a fixed order
 // through LDS: no atomics, results do not depend on scheduling.
+//
+// Cutoff guard (FP32): a truncated potential jumps by F(rc) at the cutoff, so
+// a pair whose FP32 distance rounds to the other side of rc than its FP64
+// distance changes a force by |F(rc)|.  The FP32 test therefore uses a
+// slightly wider cutoff, and every pair within kGuard (relative) of rc or of
+// the table end b is decided again in FP64 from the stored coordinates (the
+// staged slot gives the unshifted position, the shift is recovered exactly
+// as a multiple of the cell size).  In/out decisions then match FP64 on the
+// same positions; about one candidate pair in 10^6 takes this path.
 #include <algorithm>
+#include <type_traits>
 
 #include "ensemble.h"
 #include "philox.h"
@@ -34,18 +44,58 @@ constexpr int kCoefLdsMax = 16384;       // table bytes cached in LDS
 constexpr int kWaveLdsBudget = 40960;    // staging bytes of a workgroup
 
 // Pair descriptor in LDS, kPairVecs vectors per type pair:
-//   table: q0 = (a, b, alpha0, alpha1), q1 = (alpha2, n, table bits, meta bits), q2 = (r0+1, 0, 0, 0)
+//   table: q0 = (a, b, alpha0, alpha1), q1 = (alpha2, n, table bits, meta bits)
 //   DPD:   q0 = (a, b, alpha, gamma),    q1 = (sigma, 1/b, 0, meta bits)
+//   both:  q2 = (r0+1, b_hi, b_lo, 0) with b = b_hi + b_lo to 48 bits (the FP64 guard)
 // An absent pair has a = 0, b = -1, so that no distance is in range.  The
 // table bits are the LDS byte offset of the table (from the start of the
 // dynamic LDS) when the tables are cached, else its index in the global
-// coefficient array.  meta = kind << 16 | shifted << 20 | scaled << 21.
+// coefficient array.  meta = kind << 16 | shifted << 20 | scaled << 21 |
+// b_inside << 22 (b is below the cutoff, so its end needs the guard too).
 // Staged particles carry their type as the byte offset of its descriptor
 // column (type * kPairBytes), the i particle its row (type * ntypes * kPairBytes).
 constexpr int kPairVecs = 4;
 constexpr int kMetaDpd = 1 << 16;
 constexpr int kMetaShifted = 1 << 20;
 constexpr int kMetaScaled = 1 << 21;
+constexpr int kMetaBInside = 1 << 22;
+
+// FP64 squared distance of a guarded pair: pu is the partner's stored
+// (unshifted) position, pj its staged copy in the i frame.  The shift between
+// them is a whole number of cells (the staged copy is within 1e-6 of it), so
+// it is found in FP32; each axis then adds (pi - pu - shift)^2 in FP64.
+__device__ inline double guard_r2(const float4 *pu_ptr, float4 pi, float4 pj, const double *h, const float *ih) {
+    const float4 pu = *pu_ptr;
+    const float pa[3] = {pi.x, pi.y, pi.z}, ja[3] = {pj.x, pj.y, pj.z}, ua[3] = {pu.x, pu.y, pu.z};
+    double r2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float cells = rintf((ja[a] - ua[a]) * ih[a]);
+        const double e = (static_cast<double>(pa[a]) - static_cast<double>(ua[a])) - static_cast<double>(cells) * h[a];
+        r2 = __builtin_fma(e, e, r2);
+    }
+    return r2;
+}
+__device__ inline double guard_r2(const double4 *, double4, double4, const double *, const float *) { return 0.0; }
+
+// Evaluation modes of a pair (the FP32 cutoff guard).
+constexpr int kFree = 0, kDecidedIn = 1, kDecidedOut = 2;
+constexpr std::integral_constant<int, kFree> kFreeMode{};
+constexpr std::integral_constant<int, kDecidedIn> kDecidedInMode{};
+constexpr std::integral_constant<int, kDecidedOut> kDecidedOutMode{};
+
+// Squared distance with a fixed rounding (explicit FMAs, no contraction), so
+// that the pair loop and the guard's rescan classify every pair identically.
+__device__ inline float dist2(float4 a, float4 b) {
+#pragma clang fp contract(off)
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+__device__ inline double dist2(double4 a, double4 b) {
+#pragma clang fp contract(off)
+    const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+}
 
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -61,6 +111,7 @@ template <typename real>
 NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, bool radius, int max_chunk) {
     using v4 = typename V<real>::v4;
     const bool ids = dpd || flux;
+    const bool slots = std::is_same<real, float>::value;   // the cutoff guard's lookups
     NbLayout L{};
     L.waves = kNbWaves;
     L.off_coef = align16(max_nt * max_nt * kPairVecs * static_cast<int>(sizeof(v4)));
@@ -69,6 +120,8 @@ NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, b
         int b = align16((chunk + kPad) * static_cast<int>(sizeof(v4)));
         o->off_id = b;
         if (ids) b += align16(chunk * 4);
+        o->off_slot = b;
+        if (slots) b += align16(chunk * 4);
         o->off_vel = b;
         if (dpd) b += align16(chunk * static_cast<int>(sizeof(v4)));
         o->off_rad = b;
@@ -87,9 +140,10 @@ NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, b
 }
 
 template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY>
-__global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
+__device__ __forceinline__ void nonbond_body(const KP<real> &P) {
     using v4 = typename V<real>::v4;
     constexpr bool kIds = DPD || FLUX;   // pair terms that need the partner's id
+    constexpr bool kGuarded = std::is_same<real, float>::value;
     constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ FluxDesc<real> s_flux[FLUX ? kMaxFluxEntries : 1];
@@ -105,6 +159,7 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
     unsigned char *region = s_dyn + P.nb.off_waves + wid * P.nb.region;
     v4 *s_pos = reinterpret_cast<v4 *>(region);
     int *s_id = reinterpret_cast<int *>(region + P.nb.off_id);
+    int *s_slot = reinterpret_cast<int *>(region + P.nb.off_slot);
     v4 *s_vel = reinterpret_cast<v4 *>(region + P.nb.off_vel);
     real *s_rad = reinterpret_cast<real *>(region + P.nb.off_rad);
     real *s_sp = reinterpret_cast<real *>(region + P.nb.off_sp);
@@ -135,9 +190,13 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
             if (pi >= 0) {
                 const PotDesc<real> pd = P.pots[pi];
                 meta = (pd.kind == KIND_DPD ? kMetaDpd : 0) | ((pd.flags & POT_SHIFTED) ? kMetaShifted : 0) |
-                       ((pd.flags & POT_SCALED) ? kMetaScaled : 0);
+                       ((pd.flags & POT_SCALED) ? kMetaScaled : 0) |
+                       (pd.bd < sd.cutoff * (1.0 - 4.0 * kGuard) || (pd.flags & (POT_SHIFTED | POT_SCALED)) ? kMetaBInside
+                                                                                                          : 0);
                 q0.x = pd.a;
                 q0.y = pd.b;
+                q2.y = real(pd.bd);
+                q2.z = real(pd.bd - static_cast<double>(q2.y));
                 if (pd.kind == KIND_DPD) {
                     q0.z = pd.dalpha;
                     q0.w = pd.dgamma;
@@ -151,8 +210,8 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
                     q1.z = as_real_bits(CLDS ? P.nb.off_coef + (pd.coff - sd.coef_begin) * static_cast<int>(sizeof(real))
                                              : pd.coff,
                                         real(0));
-                    q2.x = pd.r0p1;
                 }
+                q2.x = pd.r0p1;
             }
             q1.w = as_real_bits(meta, real(0));
             s_pair[kPairVecs * t] = q0;
@@ -167,7 +226,12 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
             for (int t = threadIdx.x; t < sd.coef_count; t += kNbThreads) s_coef[t] = P.coef[sd.coef_begin + t];
         __syncthreads();
 
-        const real cut2 = real(sd.cutoff2);
+        // FP32: the test accepts up to rc^2 (1 + kGuard); pairs above rc^2 (1 - kGuard)
+        // are decided again in FP64.  FP64: rc^2 itself.
+        const real cut2 = kGuarded ? real(sd.cutoff2 * (1.0 + kGuard)) : real(sd.cutoff2);
+        const real cut2_lo = kGuarded ? real(sd.cutoff2 * (1.0 - kGuard)) : real(sd.cutoff2);
+        const bool binside = kGuarded && sd.binside != 0;   // a table of this system ends below the cutoff
+        const float ihf[3] = {float(1.0 / sd.h[0]), float(1.0 / sd.h[1]), float(1.0 / sd.h[2])};
         // number density W(r, rc) = (1 - 1.5 x^2 + 0.75 x^3) / (pi rc^3), x = r / rc
         // (w_cubic_spline, tf_smoothing_kernel.h:42-59), as w0 + r^2 (w2 + r w3)
         const real w0 = real(sd.w_self);
@@ -211,23 +275,46 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
 
                 // One candidate k of the staged list; ok = false makes it a no-op
                 // (k is then 0, a staged entry, so every read stays in range).
-                auto eval_pair = [&](int k, bool ok) {
+                // Modes of the FP32 cutoff guard: kFree defers a pair inside the
+                // band around the cutoff (or, for a table that ends below the
+                // cutoff, around its end b) to the rescan after the round; the
+                // rescan evaluates it with kDecidedIn / kDecidedOut once FP64 has
+                // placed it inside the cutoff, and inside or past b (past b: the
+                // number density only).
+                bool deferred = false;
+                auto eval_pair = [&](int k, bool ok, auto mode) {
+                    constexpr int kMode = decltype(mode)::value;
                     const v4 pj = s_pos[k];
                     const real dx = pi4.x - pj.x;
                     const real dy = pi4.y - pj.y;
                     const real dz = pi4.z - pj.z;
-                    const real r2 = dx * dx + dy * dy + dz * dz;
+                    const real r2 = dist2(pi4, pj);
                     // below eps the inverse distance is 1/eps (r itself only
                     // enters where it is clamped to the table's a, or through
                     // W(r) ~ W(0))
                     const real ir = rsq(fmax(r2, eps2));
                     const real r_raw = r2 * ir;
-                    nd += ok ? w0 + r2 * (w2 + r_raw * w3) : real(0);
                     const int tpb = ti_row + real_bits_as_int(pj.w);   // descriptor byte offset
                     const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + tpb);
                     const v4 q0 = qd[0];
                     const v4 q1 = qd[1];
                     const int meta = real_bits_as_int(q1.w);
+                    // distance after the radius transform of the pair's potential
+                    real rt = r_raw;
+                    if (RADIUS) {
+                        if ((meta & kMetaScaled) && !(meta & kMetaDpd))
+                            rt = r_raw / (pi4.w + s_rad[k]);
+                        else if (meta & kMetaShifted)
+                            rt = r_raw - (pi4.w + s_rad[k]) + ((meta & kMetaDpd) ? real(0) : qd[2].x);
+                    }
+                    if (kGuarded && kMode == kFree) {
+                        bool band = r2 > cut2_lo;
+                        if (binside) band = band || ((meta & kMetaBInside) && fabs(rt - q0.y) <= real(2.0 * kGuard) * q0.y);
+                        band = band && ok;
+                        deferred = deferred || band;
+                        ok = ok && !band;
+                    }
+                    nd += ok ? w0 + r2 * (w2 + r_raw * w3) : real(0);
                     int id_j = 0;
                     if (kIds) id_j = s_id[k] - sd.p_off;
                     if (FLUX && ok) {
@@ -260,12 +347,11 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
                                 qacc[fd.index_b] += qv - fd.half_decay * sb;
                         }
                     }
+                    // inside the table's range b (the FP64 decision for guarded pairs)
+                    const bool in = kMode == kFree ? ok && rt <= q0.y : (kMode == kDecidedIn);
                     if (DPD && (meta & kMetaDpd)) {
                         const real ro = r2 < eps2 ? eps_of<real>() : r_raw;
-                        real r = r_raw;
-                        if (RADIUS && (meta & kMetaShifted)) r = r - (pi4.w + s_rad[k]);
-                        const bool in = ok && r <= q0.y;
-                        r = r >= q0.x ? r : q0.x;
+                        const real r = rt >= q0.x ? rt : q0.x;
                         const real ex = dx * ir, ey = dy * ir, ez = dz * ir;
                         const v4 vj = s_vel[k];
                         const real ev = ex * (vi4.x - vj.x) + ey * (vi4.y - vj.y) + ez * (vi4.z - vj.z);
@@ -283,15 +369,7 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
                         return;
                     }
                     // Table potential (potential_eval_ex).
-                    real r = r_raw;
-                    if (RADIUS) {
-                        if (meta & kMetaScaled)
-                            r = r / (pi4.w + s_rad[k]);
-                        else if (meta & kMetaShifted)
-                            r = r - (pi4.w + s_rad[k]) + qd[2].x;
-                    }
-                    r = fmax(r, q0.x);
-                    const bool in = ok && r <= q0.y;
+                    const real r = fmax(rt, q0.x);
                     // interval in [1, n] (the clamp is exact: t is rounded down after it)
                     const int ind = static_cast<int>(clamp_real(q0.z + r * (q0.w + r * q1.x), real(1), q1.y));
                     const int tb = real_bits_as_int(q1.z);
@@ -338,14 +416,8 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
                             if (blk * 16 < my) {
                                 unsigned bits = 0;
 #pragma unroll
-                                for (int t = 0; t < 16; t++) {
-                                    const v4 pj = sp4[blk * 16 + t];
-                                    const real dx = pi4.x - pj.x;
-                                    const real dy = pi4.y - pj.y;
-                                    const real dz = pi4.z - pj.z;
-                                    const real r2 = dx * dx + dy * dy + dz * dz;
-                                    bits |= static_cast<unsigned>(r2 <= cut2) << t;
-                                }
+                                for (int t = 0; t < 16; t++)
+                                    bits |= static_cast<unsigned>(dist2(pi4, sp4[blk * 16 + t]) <= cut2) << t;
                                 const unsigned long long w = static_cast<unsigned long long>(bits) << ((blk & 3) * 16);
                                 if (blk < 4)
                                     m0 |= w;
@@ -376,8 +448,56 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
                                 k2[u] = min(base + ctz64_or_31(m0), klim);
                                 m0 &= m0 - 1ull;
                             }
-                            eval_pair(k2[0], ok2[0]);
-                            eval_pair(k2[1], ok2[1]);
+                            eval_pair(k2[0], ok2[0], kFreeMode);
+                            eval_pair(k2[1], ok2[1], kFreeMode);
+                        }
+                        // The deferred pairs, decided in FP64 (about one pair in
+                        // 10^6), in candidate order: the round's block again, with
+                        // the same tests as above.
+                        if (kGuarded && __builtin_expect(__any(deferred), 0)) {
+                            deferred = false;
+                            for (int t = 0; t < my; t++) {
+                                const int k = kb + t;
+                                const v4 pj = s_pos[k];
+                                const real r2 = dist2(pi4, pj);
+                                if (k == self_k || !(r2 <= cut2)) continue;
+                                const int tpb = ti_row + real_bits_as_int(pj.w);
+                                const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + tpb);
+                                const v4 q0 = qd[0], q2 = qd[2];
+                                const int meta = real_bits_as_int(qd[1].w);
+                                bool band = r2 > cut2_lo;
+                                if (binside) {
+                                    const real r_raw = r2 * rsq(fmax(r2, eps2));
+                                    real rt = r_raw;
+                                    if (RADIUS) {
+                                        if ((meta & kMetaScaled) && !(meta & kMetaDpd))
+                                            rt = r_raw / (pi4.w + s_rad[k]);
+                                        else if (meta & kMetaShifted)
+                                            rt = r_raw - (pi4.w + s_rad[k]) + ((meta & kMetaDpd) ? real(0) : q2.x);
+                                    }
+                                    band = band || ((meta & kMetaBInside) && fabs(rt - q0.y) <= real(2.0 * kGuard) * q0.y);
+                                }
+                                if (!band) continue;
+                                const double r2d = guard_r2(P.A.pos + s_slot[k], pi4, pj, sd.h, ihf);
+                                if (!(r2d <= sd.cutoff2)) continue;
+                                const double bd = static_cast<double>(q2.y) + static_cast<double>(q2.z);
+                                bool inb;
+                                if (RADIUS && (meta & (kMetaScaled | kMetaShifted))) {
+                                    double rd = sqrt(r2d);
+                                    const double rs = static_cast<double>(pi4.w) + static_cast<double>(s_rad[k]);
+                                    if ((meta & kMetaScaled) && !(meta & kMetaDpd))
+                                        rd = rd / rs;
+                                    else
+                                        rd = rd - rs + ((meta & kMetaDpd) ? 0.0 : static_cast<double>(q2.x));
+                                    inb = rd <= bd;
+                                } else {
+                                    inb = r2d <= bd * bd;
+                                }
+                                if (inb)
+                                    eval_pair(k, true, kDecidedInMode);
+                                else
+                                    eval_pair(k, true, kDecidedOutMode);
+                            }
                         }
                     }
                 };
@@ -437,6 +557,7 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
                                 p.w = as_real_bits((mj.x & 0xffff) * kPairBytes, real(0));
                                 s_pos[nstaged + t] = p;
                                 if (kIds) s_id[nstaged + t] = mj.y;
+                                if (kGuarded) s_slot[nstaged + t] = sj;
                                 if (DPD) s_vel[nstaged + t] = P.A.vel[sj];
                                 if (FLUX)
                                     for (int k = 0; k < S; k++)
@@ -508,13 +629,35 @@ __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
     }
 }
 
+// Kernel entries.  The FP32 table-only variants fit 96 VGPRs (5 waves per
+// SIMD) once the compiler is asked for it: the only spill sits outside the
+// cell loops.  DPD, flux and FP64 variants need more registers and run at the
+// occupancy their register count gives.
+template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY>
+__global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
+    nonbond_body<real, DPD, FLUX, RADIUS, CLDS, ENERGY>(P);
+}
+
+template <bool RADIUS, bool CLDS, bool ENERGY>
+__global__ __launch_bounds__(kNbThreads) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_nonbond_f32(KP<float> P) {
+    nonbond_body<float, false, false, RADIUS, CLDS, ENERGY>(P);
+}
+
+template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY>
+static const void *nonbond_kernel() {
+    if constexpr (std::is_same<real, float>::value && !DPD && !FLUX)
+        return reinterpret_cast<const void *>(&k_nonbond_f32<RADIUS, CLDS, ENERGY>);
+    else
+        return reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, RADIUS, CLDS, ENERGY>);
+}
+
 template <typename real, bool DPD, bool FLUX, bool RADIUS>
 static const void *nonbond_fn(bool clds, bool energy) {
     if (clds)
-        return energy ? reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, RADIUS, true, true>)
-                      : reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, RADIUS, true, false>);
-    return energy ? reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, RADIUS, false, true>)
-                  : reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, RADIUS, false, false>);
+        return energy ? nonbond_kernel<real, DPD, FLUX, RADIUS, true, true>()
+                      : nonbond_kernel<real, DPD, FLUX, RADIUS, true, false>();
+    return energy ? nonbond_kernel<real, DPD, FLUX, RADIUS, false, true>()
+                  : nonbond_kernel<real, DPD, FLUX, RADIUS, false, false>();
 }
 
 template <typename real> static const void *pick_nonbond(int key, bool clds, bool energy) {
