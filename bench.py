@@ -78,6 +78,7 @@ def time_gpu(tf, W, scene, pots, precision, steps, warmup):
     steps replayed from the captured graph, without events."""
     eng, _ = W.build_engine(scene, potentials=pots)
     eng.to_device(precision)
+    shape = eng.autotune(reps=2)   # result-invariant launch shapes (tfg_autotune)
     eng.force()
     forces = eng.particles()["f"]
     xq = eng.positions()   # the input positions as the path holds them
@@ -93,7 +94,7 @@ def time_gpu(tf, W, scene, pots, precision, steps, warmup):
     wall_graph = time.perf_counter() - t0
     eng.from_device()
     eng.close()
-    return dict(wall_s=wall, wall_graph_s=wall_graph, kernels=kt, forces=forces, positions=xq)
+    return dict(wall_s=wall, wall_graph_s=wall_graph, kernels=kt, forces=forces, positions=xq, shape=shape)
 
 
 def bench_ensemble(tf, W, nrep, steps, warmup):
@@ -104,6 +105,7 @@ def bench_ensemble(tf, W, nrep, steps, warmup):
     total = int(sum(sc["x"].shape[0] for sc in scenes))
     engines = [W.build_engine(sc)[0] for sc in scenes]
     ens = tf.Ensemble(engines, precision=tf.PREC_MIXED)
+    shape = ens.autotune(reps=2)
     ens.step(max(warmup, 1), use_graph=True)
     t0 = time.perf_counter()
     ens.step(steps, use_graph=True)
@@ -122,7 +124,7 @@ def bench_ensemble(tf, W, nrep, steps, warmup):
     return {"replicas": nrep, "particles": total, "steps": steps,
             "batched_ms_per_step": batched / steps * 1e3, "sequential_ms_per_step": sequential / steps * 1e3,
             "batched_value": total * steps / batched, "sequential_value": total * steps / sequential,
-            "speedup": sequential / batched}
+            "speedup": sequential / batched, "autotuned_workgroups": {"pair": shape[0], "stream": shape[1]}}
 
 
 def calibrate_copy(nbytes, reps):
@@ -204,6 +206,7 @@ def main():
                    "parallelism": "single GPU"},
         "phases_ms": {k: mixed["kernels"][k] for k in ("bin_avg", "nonbond_avg", "advance_avg")},
         "graph_replay_value": n * args.steps / mixed["wall_graph_s"],
+        "autotuned_workgroups": {"pair": mixed["shape"][0], "stream": mixed["shape"][1]},
         "roofline": {"bound": "hbm", "kernel": "k_nonbond", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(),
                      "bytes_per_particle": NONBOND_BYTES_PER_PARTICLE, "avg_launch_ms": nb_ms},
