@@ -38,6 +38,9 @@ typedef struct tfg_ensemble tfg_ensemble;
 /* Dynamics (source/mdcore/include/tfParticle.h:63-64). */
 #define TFG_NEWTONIAN 0
 #define TFG_OVERDAMPED 1
+/* Integrators (source/mdcore/include/tfEngine.h:89-92). */
+#define TFG_FORWARD_EULER 0
+#define TFG_RUNGE_KUTTA_4 1
 /* Flux kinds (source/mdcore/include/tfFlux.h:40-42). */
 #define TFG_FLUX_FICK 0
 #define TFG_FLUX_SECRETE 1
@@ -61,6 +64,13 @@ int tfg_engine_destroy(tfg_engine *e);
 int tfg_engine_geometry(const tfg_engine *e, int cdim[3], double h[3], int span[3]);
 /* Universe max_distance -> particle_max_dist_fraction (source/tfSimulator.cpp:661-666). */
 int tfg_engine_set_max_distance_fraction(tfg_engine *e, double fraction);
+/* Universe integrator (source/tfSimulator.cpp:218-229, EngineIntegrator of
+ * source/mdcore/include/tfEngine.h:89-92): TFG_FORWARD_EULER or
+ * TFG_RUNGE_KUTTA_4 (engine_advance_runge_kutta_4,
+ * source/mdcore/src/tf_engine_advance.cpp:499-763); any other kind is
+ * TFG_E_INVALIDARG, as the reference rejects it. */
+int tfg_engine_set_integrator(tfg_engine *e, int kind);
+int tfg_engine_integrator(const tfg_engine *e);
 /* Species count per particle (the state vector of source/state/, flat here). */
 int tfg_engine_set_nspecies(tfg_engine *e, int nspecies);
 int tfg_engine_set_dpd_noise(tfg_engine *e, int mode, uint64_t seed);

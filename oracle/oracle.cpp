@@ -23,6 +23,11 @@
 //   w_cubic_spline ........................ src/tf_smoothing_kernel.h:34-59
 //   bond_eval / angle_eval ................ src/tfBond.cpp:97-345, src/tfAngle.cpp:90-384
 //   engine_advance_forward_euler .......... src/tf_engine_advance.cpp:122-247, 309-480
+//   flux sub-steps (nr_fluxsteps > 1) ..... src/tf_engine_advance.cpp:313-323,
+//                                           src/tfEngine.cpp:1358-1376 (engine_fluxonly_eval),
+//                                           src/tf_runner_dopair.cpp:334-412, 760-812
+//   engine_advance_runge_kutta_4 .......... src/tf_engine_advance.cpp:499-763
+//   space_shuffle_local ................... src/tfSpace.cpp:274-391
 //
 // Decisions where the reference is nondeterministic or undefined:
 //   * W() uses an exact sqrt (the reference's x86 qsqrt is a 12-bit estimate).
@@ -74,6 +79,8 @@ struct Part {
     double x[3] = {0, 0, 0}, v[3] = {0, 0, 0}, f[4] = {0, 0, 0, 0};
     double imass = 1, radius = 1;
     double s[kMaxS] = {0}, q[kMaxS] = {0};
+    // Runge-Kutta state (Particle::p0, v0, xk, vk; tfParticle.h:173-176)
+    double p0[3] = {0, 0, 0}, v0[3] = {0, 0, 0}, xk[4][3] = {}, vk[4][3] = {};
 };
 
 struct Cell {
@@ -139,6 +146,9 @@ struct System {
     double origin[3], dim[3], h[3], ih[3], cutoff, cutoff2;
     int cdim[3], span[3], ncells;
     double dt, dt_flux;
+    int nr_fluxsteps = 1;
+    int integrator = 0;        // EngineIntegrator: 0 FORWARD_EULER, 1 RUNGE_KUTTA_4 (tfEngine.h:89-92)
+    bool flux_only = false;    // INTEGRATOR_FLUX_SUBSTEP: pair passes evaluate fluxes alone
     double maxdx = 0.0;      // space->maxdx (0 without Verlet lists)
     double max_frac = 0.05;  // particle_max_dist_fraction
     int nspecies = 0;
@@ -303,6 +313,12 @@ void flux_pair(const System &s, double r2, Part &p1, Part &p2) {
 // number density, flux, then potential_eval_super_ex.
 inline void pair_body(const System &s, Part &pi, Part &pj, const double dx[3], double r2, double &epot,
                       double &vir) {
+    if (s.flux_only) {
+        // runner_dopair_fluxonly / runner_doself_fluxonly: the same pair set,
+        // flux_eval_ex only (tf_runner_dopair.cpp:393-404)
+        if (!s.fluxmap[pi.type * s.max_types + pj.type].empty()) flux_pair(s, r2, pi, pj);
+        return;
+    }
     const double nd = w_cubic(r2, s.cutoff);
     pi.f[3] += nd;
     pj.f[3] += nd;
@@ -509,6 +525,7 @@ int build_space(System &s, const double origin[3], const double dim[3], const in
 void nonbond(System &s, int nthreads) {
     const double self_nd = w_cubic(0.0, s.cutoff);
     for (Cell &c : s.cells) {
+        if (s.flux_only) break;   // engine_fluxonly_eval does not run space_prepare
         c.epot = 0.0;
         for (Part &p : c.parts) {
             p.f[0] = p.f[1] = p.f[2] = 0.0;
@@ -682,6 +699,14 @@ void angles_eval(System &s, double &epot_out) {
     epot_out += epot;
 }
 
+// integrate_statevector (tfFlux.cpp:175-182); no species is KONSTANT here.
+inline void integrate_species(const System &s, Part &p) {
+    for (int k = 0; k < s.nspecies; k++) {
+        p.s[k] = std::fmax(0.0, p.s[k] + s.dt_flux * p.q[k] * 1.0);
+        p.q[k] = 0;
+    }
+}
+
 // cell_advance_forward_euler (tf_engine_advance.cpp:140-247) + Fluxes_integrate.
 void advance_cell(System &s, int cid) {
     Cell &c = s.cells[cid];
@@ -740,11 +765,7 @@ void advance_cell(System &s, int cid) {
         }
     }
     // Fluxes_integrate over the particles left in this cell.
-    for (Part &p : c.parts)
-        for (int k = 0; k < s.nspecies; k++) {
-            p.s[k] = std::fmax(0.0, p.s[k] + s.dt_flux * p.q[k] * 1.0);
-            p.q[k] = 0;
-        }
+    for (Part &p : c.parts) integrate_species(s, p);
 }
 
 double kinetic(const System &s) {
@@ -768,8 +789,18 @@ void force(System &s, int nthreads) {
     s.epot = s.epot_nb + s.epot_bond + s.epot_angle;
 }
 
-void step(System &s, int nthreads) {
+void step_euler(System &s, int nthreads) {
     s.time += 1;
+    // Flux sub-steps (tf_engine_advance.cpp:313-323): nr_fluxsteps - 1 passes
+    // of the flux term alone, each followed by Fluxes_integrate over every
+    // cell, before the force pass of the step.
+    for (int k = 0; k < s.nr_fluxsteps - 1; k++) {
+        s.flux_only = true;
+        nonbond(s, nthreads);
+        s.flux_only = false;
+        for (Cell &c : s.cells)
+            for (Part &p : c.parts) integrate_species(s, p);
+    }
     force(s, nthreads);
     // staggered order (tf_engine_advance.cpp:122-136)
     std::vector<int> ids;
@@ -799,6 +830,100 @@ void step(System &s, int nthreads) {
     s.reindex();
 }
 
+// space_shuffle_local (tfSpace.cpp:274-391): particles outside their cell
+// move to the neighbour given by delta = (x >= h) - (x < 0), shifting x and
+// the Runge-Kutta anchor p0 by -delta h; then space_cell_welcome.
+void shuffle(System &s, bool move_p0) {
+    for (Cell &c : s.cells) {
+        size_t pid = 0;
+        while (pid < c.parts.size()) {
+            Part &p = c.parts[pid];
+            int delta[3];
+            for (int k = 0; k < 3; k++) delta[k] = std::isgreaterequal(p.x[k], s.h[k]) - std::isless(p.x[k], 0.0);
+            if (delta[0] == 0 && delta[1] == 0 && delta[2] == 0) {
+                pid++;
+                continue;
+            }
+            for (int k = 0; k < 3; k++) {
+                p.x[k] -= delta[k] * s.h[k];
+                if (move_p0) p.p0[k] -= delta[k] * s.h[k];
+            }
+            Cell &dest = s.cells[s.cellid((c.loc[0] + delta[0] + s.cdim[0]) % s.cdim[0],
+                                          (c.loc[1] + delta[1] + s.cdim[1]) % s.cdim[1],
+                                          (c.loc[2] + delta[2] + s.cdim[2]) % s.cdim[2])];
+            dest.incoming.push_back(p);
+            if (pid + 1 < c.parts.size()) c.parts[pid] = c.parts.back();
+            c.parts.pop_back();
+        }
+    }
+    for (Cell &c : s.cells) {
+        for (Part &p : c.incoming) c.parts.push_back(p);
+        c.incoming.clear();
+    }
+    s.reindex();
+}
+
+// engine_advance_runge_kutta_4 (tf_engine_advance.cpp:499-763).  Four force
+// evaluations; before the second to fourth, engine_force_prep shuffles the
+// particles.  As in the reference: no velocity or displacement clamp, no
+// frozen-axis mask, no flux integration (the pending flux keeps accumulating),
+// overdamped particles keep their velocity, and the energies are those of
+// the fourth evaluation.  Stage positions farther than one cell outside
+// (x >= 2h or x <= -h) are the reference's "moving too fast" error.
+void step_rk4(System &s, int nthreads) {
+    s.time += 1;
+    const double dt = s.dt;
+    for (int stage = 0; stage < 4; stage++) {
+        if (stage > 0) shuffle(s, true);
+        force(s, nthreads);
+        for (Cell &c : s.cells)
+            for (Part &p : c.parts) {
+                const bool newt = s.types[p.type].dynamics == 0;
+                double *xk = p.xk[stage], *vk = p.vk[stage];
+                for (int k = 0; k < 3; k++) {
+                    if (stage == 0) {
+                        if (newt) {
+                            vk[k] = p.f[k] * p.imass;
+                            xk[k] = p.v[k];
+                        } else {
+                            xk[k] = p.f[k] * p.imass;
+                        }
+                        p.p0[k] = p.x[k];
+                        p.v0[k] = p.v[k];
+                        p.x[k] = p.p0[k] + 0.5 * dt * xk[k];
+                    } else if (stage < 3) {
+                        if (newt) {
+                            vk[k] = p.f[k] * p.imass;
+                            xk[k] = p.v0[k] + 0.5 * dt * p.vk[stage - 1][k];
+                        } else {
+                            xk[k] = p.f[k] * p.imass;
+                        }
+                        p.x[k] = p.p0[k] + (stage == 1 ? 0.5 * dt : dt) * xk[k];
+                    } else {
+                        if (newt) {
+                            vk[k] = p.imass * p.f[k];
+                            xk[k] = p.v0[k] + dt * p.vk[2][k];
+                            p.v[k] = p.v0[k] + (dt / 6.) * (p.vk[0][k] + 2 * p.vk[1][k] + 2 * p.vk[2][k] + vk[k]);
+                        } else {
+                            xk[k] = p.imass * p.f[k];
+                        }
+                        p.x[k] = p.p0[k] + (dt / 6.) * (p.xk[0][k] + 2 * p.xk[1][k] + 2 * p.xk[2][k] + xk[k]);
+                    }
+                }
+                for (int k = 0; k < 3; k++)
+                    if (p.x[k] >= 2.0 * s.h[k] || p.x[k] <= -s.h[k]) s.toofast = 1;
+            }
+    }
+    shuffle(s, false);
+}
+
+void step(System &s, int nthreads) {
+    if (s.integrator == 1)
+        step_rk4(s, nthreads);
+    else
+        step_euler(s, nthreads);
+}
+
 }  // namespace
 
 extern "C" {
@@ -813,6 +938,7 @@ void *or_create(const double origin[3], const double dim[3], const int cells[3],
     }
     s->dt = dt;
     s->dt_flux = dt / nr_fluxsteps;
+    s->nr_fluxsteps = nr_fluxsteps;
     s->max_frac = max_dist_fraction;
     s->max_types = max_types;
     s->potmap.assign((size_t)max_types * max_types, -1);
@@ -954,6 +1080,13 @@ int or_set_dpd_noise(void *h, int mode, unsigned long long seed) {
     System *s = static_cast<System *>(h);
     s->dpd_noise = mode;
     s->seed = seed;
+    return 0;
+}
+
+// Universe integrator kind (tfSimulator.cpp:218-229): 0 forward Euler, 1 RK4.
+int or_set_integrator(void *h, int kind) {
+    if (kind != 0 && kind != 1) return -1;
+    static_cast<System *>(h)->integrator = kind;
     return 0;
 }
 

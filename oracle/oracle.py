@@ -34,6 +34,7 @@ _SIG = {
     "or_add_bonds": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip]),
     "or_add_angles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip]),
     "or_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_ulonglong]),
+    "or_set_integrator": (ctypes.c_int, [_vp, ctypes.c_int]),
     "or_set_sweep_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "or_get_sweep_mode": (ctypes.c_int, [_vp]),
     "or_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
@@ -100,10 +101,12 @@ class Oracle:
         c = np.ascontiguousarray(scene["cells"], dtype=np.int32)
         ntypes = max(8, len(scene["types"]))
         self.nspecies = scene["nspecies"]
-        self._h = L.or_create(_d(o), _d(d), c.ctypes.data_as(_ip), scene["cutoff"], ntypes, scene["dt"], 1,
-                              max_distance_fraction, self.nspecies)
+        self._h = L.or_create(_d(o), _d(d), c.ctypes.data_as(_ip), scene["cutoff"], ntypes, scene["dt"],
+                              int(scene.get("nr_fluxsteps", 1)), max_distance_fraction, self.nspecies)
         if not self._h:
             raise ValueError("oracle rejected the geometry")
+        if L.or_set_integrator(self._h, int(scene.get("integrator", 0))) != 0:
+            raise ValueError("unknown integrator kind")
         for t in scene["types"]:
             L.or_add_type(self._h, t["mass"], t["radius"], t["dynamics"])
         ids = []
@@ -158,6 +161,11 @@ class Oracle:
         """0 exhaustive, 1 signed sorted sweep, 2 the reference's unsigned sweep."""
         if lib().or_set_sweep_mode(self._h, mode) != 0:
             raise ValueError(mode)
+
+    def set_integrator(self, kind):
+        """0 forward Euler, 1 Runge-Kutta 4."""
+        if lib().or_set_integrator(self._h, int(kind)) != 0:
+            raise ValueError(kind)
 
     @property
     def sweep_mode(self):

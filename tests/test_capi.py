@@ -93,3 +93,16 @@ def test_geometry_and_insertion(tf):
         ind = (x * (1.0 / h[0])).astype(int)
         assert p["cid"][k] == (ind[0] * 28 + ind[1]) * 28 + ind[2]
         assert np.allclose(p["x"][k], x - ind * h[0], atol=1e-12)
+
+
+def test_integrator_setting(tf):
+    """Universe integrator kinds 0 and 1 (tfSimulator.cpp:218-229); others are rejected."""
+    eng = tf.Engine(dim=[10.0] * 3, cells=[3] * 3, cutoff=1.0)
+    assert eng.integrator == tf.FORWARD_EULER
+    eng.integrator = tf.RUNGE_KUTTA_4
+    assert eng.integrator == tf.RUNGE_KUTTA_4
+    with pytest.raises(tf.TfgError):
+        eng.integrator = 2
+    assert eng.integrator == tf.RUNGE_KUTTA_4
+    e2 = tf.Engine(dim=[10.0] * 3, cells=[3] * 3, cutoff=1.0, integrator=tf.RUNGE_KUTTA_4, nr_fluxsteps=3)
+    assert e2.integrator == tf.RUNGE_KUTTA_4

@@ -27,6 +27,8 @@ DPD_NOISE_UNIT = 0
 DPD_NOISE_PHILOX = 1
 PREC_MIXED = 0
 PREC_FP64 = 1
+FORWARD_EULER = 0
+RUNGE_KUTTA_4 = 1
 
 FROZEN_X = 1 << 3
 FROZEN_Y = 1 << 4
@@ -58,6 +60,8 @@ _SIGNATURES = {
     "tfg_engine_geometry": (ctypes.c_int, [_vp, _ip, _dp, _ip]),
     "tfg_engine_set_max_distance_fraction": (ctypes.c_int, [_vp, ctypes.c_double]),
     "tfg_engine_set_nspecies": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfg_engine_set_integrator": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfg_engine_integrator": (ctypes.c_int, [_vp]),
     "tfg_engine_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64]),
     "tfg_engine_time": (ctypes.c_int64, [_vp]),
     "tfg_engine_addtype": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
@@ -226,7 +230,7 @@ class Engine:
     """One simulated system: engine_init + the Universe configuration."""
 
     def __init__(self, dim, cells, cutoff, dt=0.01, max_types=8, nr_fluxsteps=1, origin=(0.0, 0.0, 0.0),
-                 nspecies=0, max_distance_fraction=None):
+                 nspecies=0, max_distance_fraction=None, integrator=FORWARD_EULER):
         L = lib()
         self._h = _vp()
         o = _f64(origin, 3)
@@ -240,6 +244,8 @@ class Engine:
             _check(L.tfg_engine_set_nspecies(self._h, nspecies))
         if max_distance_fraction is not None:
             _check(L.tfg_engine_set_max_distance_fraction(self._h, max_distance_fraction))
+        if integrator != FORWARD_EULER:
+            self.integrator = integrator
 
     def close(self):
         if getattr(self, "_h", None):
@@ -270,6 +276,15 @@ class Engine:
 
     def add_flux(self, kind, ta, tb, sa, sb, k, decay=0.0, target=0.0, cutoff=0.0):
         _check(lib().tfg_engine_addflux(self._h, kind, ta, tb, sa, sb, k, decay, target, cutoff))
+
+    @property
+    def integrator(self):
+        """FORWARD_EULER or RUNGE_KUTTA_4 (Universe config `integrator`)."""
+        return _check(lib().tfg_engine_integrator(self._h))
+
+    @integrator.setter
+    def integrator(self, kind):
+        _check(lib().tfg_engine_set_integrator(self._h, int(kind)))
 
     def set_dpd_noise(self, mode, seed=0):
         _check(lib().tfg_engine_set_dpd_noise(self._h, mode, seed))

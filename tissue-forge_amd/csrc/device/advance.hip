@@ -1,5 +1,5 @@
-// Bonded forces and the forward-Euler update (reads buffer A, writes B in
-// the same order), plus the FP64 energy and species-total reductions.
+// Bonded forces and the particle update (reads buffer A, writes B in the
+// same order), plus the FP64 energy and species-total reductions.
 //
 // Bonds follow bond_eval (tfBond.cpp:97-345), angles angle_eval
 // (tfAngle.cpp:90-384), gathered per particle over a static incidence list
@@ -7,8 +7,10 @@
 // cell_advance_forward_euler (tf_engine_advance.cpp:140-247): Newtonian or
 // overdamped Euler with the per-axis displacement clamp, cell migration by
 // delta = (x >= h) - (x < 0) and Fluxes_integrate (tfFlux.cpp:175-202) for
-// particles that stay in their cell.  The new cell counts for the next
-// binning are accumulated here, one atomic per run of equal cell ids.
+// particles that stay in their cell.  The Runge-Kutta stages follow
+// engine_advance_runge_kutta_4 (tf_engine_advance.cpp:499-763).  The new cell
+// counts for the next binning are accumulated here, one atomic per run of
+// equal cell ids.
 #include "ensemble.h"
 
 namespace tfg {
@@ -16,6 +18,15 @@ namespace tfg {
 constexpr int kEnergyTile = 4096;   // particles per energy-partial block
 
 __device__ inline int wrap_shift(int s) { return s > 1 ? -1 : (s < -1 ? 1 : s); }
+
+template <typename real> __device__ inline typename V<real>::v4 v4_of(real x, real y, real z) {
+    typename V<real>::v4 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    r.w = real(0);
+    return r;
+}
 
 template <typename real>
 __device__ inline void eval_table_r(const PotDesc<real> &pd, const real *coef, real r, real *e, real *eff_out) {
@@ -31,7 +42,9 @@ __device__ inline void eval_table_r(const PotDesc<real> &pd, const real *coef, r
     *eff_out = eff * cc[1];
 }
 
-template <typename real>
+// RK4 selects the instance with the Runge-Kutta stages (update 2..5); the
+// other serves updates 0 and 1.
+template <typename real, bool RK4>
 __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
     using v4 = typename V<real>::v4;
     const int S = P.S;
@@ -148,7 +161,7 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
             }
             newc = c;
             bool moved = false;
-            if (update) {
+            if (!RK4 && update == 1) {
                 const bool frozen_all = (flags & (PFLAG_FROZEN_X | PFLAG_FROZEN_Y | PFLAG_FROZEN_Z)) ==
                                         (PFLAG_FROZEN_X | PFLAG_FROZEN_Y | PFLAG_FROZEN_Z);
                 if (!(flags & PFLAG_CLUSTER) && !frozen_all) {
@@ -193,6 +206,99 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                     v.y = vv[1];
                     v.z = vv[2];
                 }
+            } else if (RK4) {
+                // Runge-Kutta stage st (0..3).  As in the reference: no
+                // clamp, no frozen-axis mask; overdamped particles keep their
+                // velocity; stages 0-2 leave the particle at p0 + c dt xk and
+                // move it (and p0) to the neighbour cell it entered, as the
+                // next engine_force_prep's shuffle does (tfSpace.cpp:300-345).
+#pragma clang fp contract(off)
+                const int st = update - 2;
+                const real dt = real(sd.dt);
+                const real hdt = real(0.5 * sd.dt);
+                const real dt6 = real(sd.dt / 6.);
+                const real im = P.A.imass[i];
+                const bool newt = td.dynamics == DYN_NEWTONIAN;
+                real xv[3] = {p.x, p.y, p.z}, vv[3] = {v.x, v.y, v.z};
+                const real fv[3] = {f.x, f.y, f.z};
+                real p0[3], v0[3], sx[3], sv[3], vkp[3] = {0, 0, 0};
+                if (st == 0) {
+                    for (int k = 0; k < 3; k++) {
+                        p0[k] = xv[k];
+                        v0[k] = vv[k];
+                        sx[k] = sv[k] = real(0);
+                    }
+                } else {
+                    const v4 a0 = P.A.p0[i], b0 = P.A.v0[i], cx = P.A.sx[i], cv = P.A.sv[i], dk = P.A.vk[i];
+                    p0[0] = a0.x; p0[1] = a0.y; p0[2] = a0.z;
+                    v0[0] = b0.x; v0[1] = b0.y; v0[2] = b0.z;
+                    sx[0] = cx.x; sx[1] = cx.y; sx[2] = cx.z;
+                    sv[0] = cv.x; sv[1] = cv.y; sv[2] = cv.z;
+                    vkp[0] = dk.x; vkp[1] = dk.y; vkp[2] = dk.z;
+                }
+                real vkn[3];
+                int delta[3];
+                bool far = false;
+                for (int k = 0; k < 3; k++) {
+                    real xk, vk = real(0);
+                    if (st == 0) {
+                        if (newt) {
+                            vk = fv[k] * im;
+                            xk = vv[k];
+                        } else {
+                            xk = fv[k] * im;
+                        }
+                        sx[k] = xk;
+                        sv[k] = vk;
+                        xv[k] = p0[k] + hdt * xk;
+                    } else if (st < 3) {
+                        if (newt) {
+                            vk = fv[k] * im;
+                            xk = v0[k] + hdt * vkp[k];
+                        } else {
+                            xk = fv[k] * im;
+                        }
+                        sx[k] = sx[k] + real(2) * xk;
+                        sv[k] = sv[k] + real(2) * vk;
+                        xv[k] = p0[k] + (st == 1 ? hdt : dt) * xk;
+                    } else {
+                        if (newt) {
+                            vk = im * fv[k];
+                            xk = v0[k] + dt * vkp[k];
+                            vv[k] = v0[k] + dt6 * (sv[k] + vk);
+                        } else {
+                            xk = im * fv[k];
+                        }
+                        xv[k] = p0[k] + dt6 * (sx[k] + xk);
+                    }
+                    vkn[k] = vk;
+                    // x + foff >= h, x + foff < 0 in the stored frame
+                    delta[k] = (xv[k] >= real(sd.h[k] - sd.foff[k])) - (xv[k] < real(-sd.foff[k]));
+                    far = far || xv[k] >= real(2.0 * sd.h[k] - sd.foff[k]) || xv[k] <= real(-sd.h[k] - sd.foff[k]);
+                }
+                if (far) atomicOr(P.err, 1);
+                if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
+                    moved = true;
+                    for (int k = 0; k < 3; k++) {
+                        xv[k] = real(static_cast<double>(xv[k]) - delta[k] * sd.h[k]);
+                        p0[k] = real(static_cast<double>(p0[k]) - delta[k] * sd.h[k]);
+                        loc[k] = (loc[k] + delta[k] + sd.cdim[k]) % sd.cdim[k];
+                    }
+                    newc = sd.cell_off + (loc[0] * sd.cdim[1] + loc[1]) * sd.cdim[2] + loc[2];
+                }
+                p.x = xv[0];
+                p.y = xv[1];
+                p.z = xv[2];
+                v.x = vv[0];
+                v.y = vv[1];
+                v.z = vv[2];
+                if (st < 3) {
+                    P.B.p0[i] = v4_of<real>(p0[0], p0[1], p0[2]);
+                    P.B.v0[i] = v4_of<real>(v0[0], v0[1], v0[2]);
+                    P.B.sx[i] = v4_of<real>(sx[0], sx[1], sx[2]);
+                    P.B.sv[i] = v4_of<real>(sv[0], sv[1], sv[2]);
+                    P.B.vk[i] = v4_of<real>(vkn[0], vkn[1], vkn[2]);
+                }
             }
             P.B.pos[i] = p;
             P.B.vel[i] = v;
@@ -204,7 +310,7 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
             for (int k = 0; k < S; k++) {
                 const size_t o = (size_t)k * P.np + i;
                 real s = P.A.sp[o], qq = P.A.q[o];
-                if (update && !moved) {
+                if (!RK4 && update == 1 && !moved) {
                     s = s + real(sd.dt_flux) * qq;
                     s = s > real(0) ? s : real(0);
                     qq = real(0);
@@ -285,8 +391,30 @@ __global__ __launch_bounds__(256) void k_species_total(const SysDesc *sys, const
     }
 }
 
+// Fluxes_integrate over every particle (tfFlux.cpp:175-202), in place on
+// buffer A: the flux sub-steps of tf_engine_advance.cpp:313-323.
+template <typename real> __global__ __launch_bounds__(256) void k_flux_integrate(KP<real> P) {
+    const size_t n = (size_t)P.np * P.S;
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (size_t)gridDim.x * blockDim.x) {
+        const int i = static_cast<int>(o % P.np);
+        const SysDesc &sd = P.sys[P.cell_sys[P.A.meta[i].z]];
+        real s = P.A.sp[o] + real(sd.dt_flux) * P.A.q[o];
+        P.A.sp[o] = s > real(0) ? s : real(0);
+        P.A.q[o] = real(0);
+    }
+}
+
+template <typename real> int run_flux_integrate(DeviceEnsemble *E, const KP<real> &P) {
+    if (E->S == 0) return TFG_OK;
+    hipLaunchKernelGGL(k_flux_integrate<real>, dim3(E->stream_grid()), dim3(256), 0, E->stream, P);
+    return launch_error("flux integrate");
+}
+
 template <typename real> int run_advance(DeviceEnsemble *E, const KP<real> &P, int update) {
-    hipLaunchKernelGGL(k_advance<real>, dim3(E->stream_grid()), dim3(256), 0, E->stream, P, update);
+    if (update >= 2)
+        hipLaunchKernelGGL((k_advance<real, true>), dim3(E->stream_grid()), dim3(256), 0, E->stream, P, update);
+    else
+        hipLaunchKernelGGL((k_advance<real, false>), dim3(E->stream_grid()), dim3(256), 0, E->stream, P, update);
     return launch_error("advance");
 }
 
@@ -301,6 +429,8 @@ template <typename real> int run_energy(DeviceEnsemble *E, const KP<real> &P) {
     return launch_error("energy");
 }
 
+template int run_flux_integrate<float>(DeviceEnsemble *, const KP<float> &);
+template int run_flux_integrate<double>(DeviceEnsemble *, const KP<double> &);
 template int run_advance<float>(DeviceEnsemble *, const KP<float> &, int);
 template int run_advance<double>(DeviceEnsemble *, const KP<double> &, int);
 template int run_energy<float>(DeviceEnsemble *, const KP<float> &);

@@ -188,6 +188,13 @@ __global__ void k_permute(KP<real> P) {
             P.A.sp[(size_t)k * P.np + i] = P.B.sp[(size_t)k * P.np + src];
             P.A.q[(size_t)k * P.np + i] = P.B.q[(size_t)k * P.np + src];
         }
+        if (P.A.p0) {   // Runge-Kutta stage state
+            P.A.p0[i] = P.B.p0[src];
+            P.A.v0[i] = P.B.v0[src];
+            P.A.sx[i] = P.B.sx[src];
+            P.A.sv[i] = P.B.sv[src];
+            P.A.vk[i] = P.B.vk[src];
+        }
         P.slot_of[m.y] = i;
     }
 }

@@ -78,6 +78,11 @@ template <typename real> struct PArrays {
     real *imass;
     real *sp;                    // species values, S per particle (species-major)
     real *q;                     // pending species flux
+    // Runge-Kutta stage state (null with forward Euler): anchor position p0
+    // (stored frame), start velocity v0, running sums of the stage
+    // displacement rates xk and accelerations vk (weights 1, 2, 2), and the
+    // last stage's vk.
+    typename V<real>::v4 *p0, *v0, *sx, *sv, *vk;
 };
 
 // Dynamic LDS layout of the pair kernel (bytes): pair descriptors at 0,

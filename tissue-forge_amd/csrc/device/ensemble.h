@@ -51,6 +51,8 @@ struct DeviceEnsemble {
     int nsys = 0, np = 0, ncells = 0, S = 0;
     int max_sys_blocks = 1;   // energy partial blocks per system
     bool has_dpd = false, has_flux = false, has_radius = false;
+    int integrator = INTEG_FORWARD_EULER;   // shared by every system of the ensemble
+    int nr_fluxsteps = 1;                   // shared as well
     int coef_lds_bytes = 0;   // table bytes cached in LDS by the pair kernel (0: read from global)
     NbLayout nb{};            // LDS layout of the pair kernel
     int nonbond_resident = 0; // pair-kernel workgroups resident on the device at once
@@ -73,6 +75,7 @@ struct DeviceEnsemble {
     TypeDesc *d_types = nullptr;
     void *pos[2] = {nullptr, nullptr}, *vel[2] = {nullptr, nullptr}, *imass[2] = {nullptr, nullptr};
     void *sp[2] = {nullptr, nullptr}, *qq[2] = {nullptr, nullptr};
+    void *rk[2][5] = {{nullptr}};   // Runge-Kutta p0, v0, sx, sv, vk per buffer
     int4 *meta[2] = {nullptr, nullptr};
     void *force = nullptr;
     int *slot_of = nullptr, *cnt = nullptr, *fill = nullptr, *start = nullptr, *perm = nullptr, *bsum = nullptr;
@@ -119,7 +122,10 @@ int run_count(DeviceEnsemble *E);
 template <typename real> int run_nonbond(DeviceEnsemble *E, const KP<real> &P);
 // Chooses the LDS layout and the resident grid of the pair kernel.
 template <typename real> int nonbond_setup(DeviceEnsemble *E, int max_types, int max_coef);
+// update: 0 forces only, 1 forward-Euler update, 2..5 Runge-Kutta stage 1..4.
 template <typename real> int run_advance(DeviceEnsemble *E, const KP<real> &P, int update);
+// Fluxes_integrate over every particle of buffer A (the flux sub-steps).
+template <typename real> int run_flux_integrate(DeviceEnsemble *E, const KP<real> &P);
 template <typename real> int run_energy(DeviceEnsemble *E, const KP<real> &P);
 
 }  // namespace tfg

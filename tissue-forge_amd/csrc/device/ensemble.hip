@@ -37,6 +37,11 @@ template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy) {
         arr[s]->imass = static_cast<real *>(E->imass[s]);
         arr[s]->sp = static_cast<real *>(E->sp[s]);
         arr[s]->q = static_cast<real *>(E->qq[s]);
+        arr[s]->p0 = static_cast<v4 *>(E->rk[s][0]);
+        arr[s]->v0 = static_cast<v4 *>(E->rk[s][1]);
+        arr[s]->sx = static_cast<v4 *>(E->rk[s][2]);
+        arr[s]->sv = static_cast<v4 *>(E->rk[s][3]);
+        arr[s]->vk = static_cast<v4 *>(E->rk[s][4]);
     }
     P.force = static_cast<v4 *>(E->force);
     P.slot_of = E->slot_of;
@@ -72,19 +77,40 @@ __global__ void k_tick(long long *step) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
-// One step: tick, bin, nonbond, advance (energy pass when requested).
+// One step (engine_step): tick, then
+//   forward Euler   bin, nr_fluxsteps - 1 flux sub-steps (pair pass +
+//                   Fluxes_integrate over every particle), nonbond, advance
+//                   (tf_engine_advance.cpp:309-480);
+//   Runge-Kutta 4   four rounds of bin, nonbond, stage update
+//                   (tf_engine_advance.cpp:499-763);
+// and the energy pass when requested.  The phase events time the last
+// bin / nonbond / advance round.
 template <typename real> static int enqueue_step(DeviceEnsemble *E, bool energy, hipEvent_t *ev) {
-    KP<real> P = make_params<real>(E, energy ? 1 : 0);
+    const KP<real> P = make_params<real>(E, 0);
+    const KP<real> Pe = make_params<real>(E, energy ? 1 : 0);
     int rc;
-    if (ev) TFG_HIP(hipEventRecord(ev[0], E->stream));
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, E->stream, E->d_step);
-    if ((rc = run_bin<real>(E, P)) != TFG_OK) return rc;
-    if (ev) TFG_HIP(hipEventRecord(ev[1], E->stream));
-    if ((rc = run_nonbond<real>(E, P)) != TFG_OK) return rc;
-    if (ev) TFG_HIP(hipEventRecord(ev[2], E->stream));
-    if ((rc = run_advance<real>(E, P, 1)) != TFG_OK) return rc;
-    if (ev) TFG_HIP(hipEventRecord(ev[3], E->stream));
-    if (energy && (rc = run_energy<real>(E, P)) != TFG_OK) return rc;
+    const bool rk4 = E->integrator == INTEG_RUNGE_KUTTA_4;
+    const int rounds = rk4 ? 4 : 1;
+    for (int st = 0; st < rounds; st++) {
+        const bool last = st == rounds - 1;
+        const KP<real> &Q = last ? Pe : P;
+        if (ev && last) TFG_HIP(hipEventRecord(ev[0], E->stream));
+        if ((rc = run_bin<real>(E, Q)) != TFG_OK) return rc;
+        if (ev && last) TFG_HIP(hipEventRecord(ev[1], E->stream));
+        // The sub-step pair pass is the full pair kernel; only its pending
+        // flux is kept (the forces are rewritten by the step's own pass).
+        if (!rk4)
+            for (int k = 0; k < E->nr_fluxsteps - 1; k++) {
+                if (E->has_flux && (rc = run_nonbond<real>(E, P)) != TFG_OK) return rc;
+                if ((rc = run_flux_integrate<real>(E, P)) != TFG_OK) return rc;
+            }
+        if ((rc = run_nonbond<real>(E, Q)) != TFG_OK) return rc;
+        if (ev && last) TFG_HIP(hipEventRecord(ev[2], E->stream));
+        if ((rc = run_advance<real>(E, Q, rk4 ? 2 + st : 1)) != TFG_OK) return rc;
+        if (ev && last) TFG_HIP(hipEventRecord(ev[3], E->stream));
+    }
+    if (energy && (rc = run_energy<real>(E, Pe)) != TFG_OK) return rc;
     return TFG_OK;
 }
 
@@ -115,10 +141,17 @@ int ensemble_create(DeviceEnsemble **out, Engine *const *engines, int n, int pre
         if (engines[k]->device) return set_error(TFG_E_STATE, "engine already on a device");
         for (int j = 0; j < k; j++)
             if (engines[j] == engines[k]) return set_error(TFG_E_INVALIDARG, "engine listed twice");
+        // The step sequence is shared: one integrator and flux sub-step count.
+        if (engines[k]->integrator != engines[0]->integrator)
+            return set_error(TFG_E_INVALIDARG, "ensemble members must use the same integrator");
+        if (engines[k]->nr_fluxsteps != engines[0]->nr_fluxsteps)
+            return set_error(TFG_E_INVALIDARG, "ensemble members must use the same number of flux steps");
     }
     DeviceEnsemble *E = new DeviceEnsemble();
     E->precision = precision;
     E->engines.assign(engines, engines + n);
+    E->integrator = engines[0]->integrator;
+    E->nr_fluxsteps = engines[0]->nr_fluxsteps;
     int rc = precision == PREC_MIXED ? upload<float>(E) : upload<double>(E);
     if (rc != TFG_OK) {
         delete E;
@@ -373,6 +406,8 @@ template <typename real> static int upload(DeviceEnsemble *E) {
         ALLOC(E->imass[s], real, np);
         ALLOC(E->sp[s], real, (size_t)np * S);
         ALLOC(E->qq[s], real, (size_t)np * S);
+        if (E->integrator == INTEG_RUNGE_KUTTA_4)
+            for (int a = 0; a < 5; a++) ALLOC(E->rk[s][a], v4, np);
     }
     ALLOC(E->force, v4, np);
     ALLOC(E->slot_of, int, np);

@@ -128,6 +128,21 @@ int tfg_engine_set_max_distance_fraction(tfg_engine *h, double fraction) {
     }));
 }
 
+int tfg_engine_set_integrator(tfg_engine *h, int kind) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    if (kind != INTEG_FORWARD_EULER && kind != INTEG_RUNGE_KUTTA_4)
+        return set_error(TFG_E_INVALIDARG, "invalid integrator kind: " + std::to_string(kind));
+    TFG_GUARD(return with_refresh(h, [&] {
+        h->e.integrator = kind;
+        return TFG_OK;
+    }));
+}
+
+int tfg_engine_integrator(const tfg_engine *h) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    return h->e.integrator;
+}
+
 int tfg_engine_set_nspecies(tfg_engine *h, int nspecies) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
     if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");

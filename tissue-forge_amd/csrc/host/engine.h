@@ -31,6 +31,9 @@ enum ParticleFlags : uint32_t {
 enum Dynamics : int { DYN_NEWTONIAN = 0, DYN_OVERDAMPED = 1 };   // tfParticle.h:63-64
 enum FluxKind : int { FLUX_FICK = 0, FLUX_SECRETE = 1, FLUX_UPTAKE = 2 };   // tfFlux.h:40-42
 enum DpdNoise : int { DPD_NOISE_UNIT = 0, DPD_NOISE_PHILOX = 1 };
+// EngineIntegrator (tfEngine.h:89-92), chosen by the Universe configuration
+// (tfSimulator.cpp:218-229).
+enum Integrator : int { INTEG_FORWARD_EULER = 0, INTEG_RUNGE_KUTTA_4 = 1 };
 
 struct ParticleType {
     std::string name;
@@ -101,6 +104,7 @@ class Engine {
     int nr_fluxsteps = 1;
     double dt_flux = 0.01;
     double max_dist_fraction = 0.05;   // tfEngine.cpp:1865
+    int integrator = INTEG_FORWARD_EULER;
     int64_t time = 0;                  // step counter (engine->time)
     int dpd_noise = DPD_NOISE_UNIT;
     uint64_t dpd_seed = 0;

@@ -156,7 +156,8 @@ def build_engine(scene, potentials=None, max_distance_fraction=None):
     pots = potentials if potentials is not None else [make_potential(p) for p in scene["pots"]]
     eng = Engine(dim=scene["dim"], cells=scene["cells"], cutoff=scene["cutoff"], dt=scene["dt"],
                  max_types=max(8, len(scene["types"])), nspecies=scene["nspecies"],
-                 max_distance_fraction=max_distance_fraction)
+                 max_distance_fraction=max_distance_fraction, nr_fluxsteps=int(scene.get("nr_fluxsteps", 1)),
+                 integrator=int(scene.get("integrator", 0)))
     for t in scene["types"]:
         eng.add_type(t["name"], mass=t["mass"], radius=t["radius"], dynamics=t["dynamics"])
     for pi, a, b in scene["bind"]:
