@@ -41,6 +41,19 @@ typedef struct tfg_ensemble tfg_ensemble;
 /* Integrators (source/mdcore/include/tfEngine.h:89-92). */
 #define TFG_FORWARD_EULER 0
 #define TFG_RUNGE_KUTTA_4 1
+/* Boundary condition kinds (BoundaryConditionKind,
+ * source/mdcore/include/tfBoundaryConditions.h:38-46) and faces. */
+#define TFG_BOUNDARY_VELOCITY 1u
+#define TFG_BOUNDARY_PERIODIC 2u
+#define TFG_BOUNDARY_FREESLIP 4u
+#define TFG_BOUNDARY_POTENTIAL 8u
+#define TFG_BOUNDARY_NO_SLIP 16u
+#define TFG_FACE_LEFT 0
+#define TFG_FACE_RIGHT 1
+#define TFG_FACE_FRONT 2
+#define TFG_FACE_BACK 3
+#define TFG_FACE_BOTTOM 4
+#define TFG_FACE_TOP 5
 /* Flux kinds (source/mdcore/include/tfFlux.h:40-42). */
 #define TFG_FLUX_FICK 0
 #define TFG_FLUX_SECRETE 1
@@ -71,6 +84,21 @@ int tfg_engine_set_max_distance_fraction(tfg_engine *e, double fraction);
  * TFG_E_INVALIDARG, as the reference rejects it. */
 int tfg_engine_set_integrator(tfg_engine *e, int kind);
 int tfg_engine_integrator(const tfg_engine *e);
+/* One face's boundary condition: BoundaryCondition::init
+ * (source/mdcore/src/tfBoundaryConditions.cpp:151-171; NO_SLIP becomes a
+ * velocity face with zero velocity), restore and the wall radius
+ * (tfBoundaryConditions.h:57-74).  Faces start periodic; when one face of an
+ * axis is periodic both are (check_periodicy, tfBoundaryConditions.cpp:192-217).
+ * Kinds are bit sets of TFG_BOUNDARY_*; "potential" faces of the reference's
+ * string form are TFG_BOUNDARY_POTENTIAL | TFG_BOUNDARY_FREESLIP (:112).
+ * Resetting faces are not supported (TFG_E_INVALIDARG). */
+int tfg_engine_set_boundary(tfg_engine *e, int face, uint32_t kind, const double velocity[3],
+                            double restore, double radius);
+/* BoundaryCondition::set_potential / BoundaryConditions::set_potential
+ * (source/mdcore/src/tfBoundaryConditions.cpp:504-525): the wall potential of
+ * a particle type on one face, or on all six with face = -1; NULL clears it.
+ * Evaluated by boundary_eval (source/mdcore/src/tf_boundary_eval.h:204-254). */
+int tfg_engine_set_wall_potential(tfg_engine *e, int face, int type, const tfg_potential *p);
 /* Species count per particle (the state vector of source/state/, flat here). */
 int tfg_engine_set_nspecies(tfg_engine *e, int nspecies);
 int tfg_engine_set_dpd_noise(tfg_engine *e, int mode, uint64_t seed);

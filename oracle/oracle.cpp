@@ -28,6 +28,12 @@
 //                                           src/tf_runner_dopair.cpp:334-412, 760-812
 //   engine_advance_runge_kutta_4 .......... src/tf_engine_advance.cpp:499-763
 //   space_shuffle_local ................... src/tfSpace.cpp:274-391
+//   boundary conditions: cell flags and
+//     non-periodic task list .............. src/tfSpace.cpp:784-831, 905-975
+//   boundary_eval (wall potentials) ....... src/tf_boundary_eval.h:143-254, src/tf_runner_dopair.cpp:594-596
+//   dpd_boundary_eval ..................... src/tf_dpd_eval.h:78-124
+//   apply_update_pos_vel (reflection) ..... src/tf_boundary_eval.h:45-131, src/tf_engine_advance.cpp:193-200
+//   BoundaryCondition kinds ............... src/tfBoundaryConditions.cpp:135-217 (NO_SLIP, check_periodicy)
 //
 // Decisions where the reference is nondeterministic or undefined:
 //   * W() uses an exact sqrt (the reference's x86 qsqrt is a 12-bit estimate).
@@ -86,6 +92,7 @@ struct Part {
 struct Cell {
     int loc[3];
     double origin[3];
+    unsigned active = 0;   // cell_active_* : bit f for face f on the domain boundary
     std::vector<Part> parts, incoming;
     std::vector<unsigned> sortlist;   // 13 * count
     double epot = 0;
@@ -100,6 +107,25 @@ struct Task {
 struct Type {
     double mass, imass, radius;
     int dynamics;
+};
+
+// BoundaryConditionKind (tfBoundaryConditions.h:38-46).
+enum : unsigned {
+    BC_VELOCITY = 1u << 0,
+    BC_PERIODIC = 1u << 1,
+    BC_FREESLIP = 1u << 2,
+    BC_POTENTIAL = 1u << 3,
+    BC_NO_SLIP = 1u << 4,
+    BC_ACTIVE = BC_FREESLIP | BC_VELOCITY | BC_POTENTIAL,
+};
+
+// One face: 0 left (-x), 1 right (+x), 2 front (-y), 3 back (+y),
+// 4 bottom (-z), 5 top (+z).
+struct Boundary {
+    unsigned kind = BC_PERIODIC;
+    double vel[3] = {0, 0, 0};
+    double restore = 1.0;
+    double radius = 0.0;
 };
 
 // Unit directions of the 13 sort lists (src/tfSpace_cell.cpp:81-95).
@@ -177,6 +203,10 @@ struct System {
     int sweep_mode = 1;
     double epot = 0, epot_nb = 0, epot_bond = 0, epot_angle = 0, ekin = 0, virial = 0;
     std::atomic<int> toofast{0};
+    Boundary bc_set[6];             // the faces as set
+    Boundary bc[6];                 // in effect (after check_periodicy)
+    unsigned periodic = 7;          // bit k: axis k periodic
+    std::vector<int> wallpot;       // 6 * max_types: potential of (face, type) or -1
 
     int cellid(int i, int j, int k) const { return (i * cdim[1] + j) * cdim[2] + k; }
 
@@ -364,12 +394,64 @@ inline void pair_body(const System &s, Part &pi, Part &pj, const double dx[3], d
     vir -= f * r2;
 }
 
+// dpd_boundary_eval (tf_dpd_eval.h:78-124): the wall as a particle of
+// radius rj moving with the wall velocity; the random term is the constant
+// sigma * omega / sqrt(dt) in every noise mode, and the energy is 0.
+void dpd_wall(const System &s, const Pot &p, Part &pi, double rj, const double *wv, const double dx[3], double r2) {
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double delta = 1.0 / std::sqrt(s.dt);
+    double r = std::sqrt(r2);
+    const double ro = r < eps ? eps : r;
+    r = (p.flags & POT_SHIFTED) ? r - (pi.radius + rj) : r;
+    if (r > p.b) return;
+    r = r >= p.a ? r : p.a;
+    const double e[3] = {dx[0] / ro, dx[1] / ro, dx[2] / ro};
+    const double v[3] = {pi.v[0] - wv[0], pi.v[1] - wv[1], pi.v[2] - wv[2]};
+    const double om = r < 0.0 ? 1.0 : (1 - r / p.b);
+    const double fc = p.dalpha * om;
+    const double fd = -p.dgamma * om * om * (e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
+    const double fr = p.dsigma * om * delta;
+    const double f = fc + fd + fr;
+    for (int k = 0; k < 3; k++) pi.f[k] = pi.f[k] + f * e[k];
+}
+
+// boundary_eval (tf_boundary_eval.h:204-254): wall potentials of the active
+// faces of the particle's cell, in the order left, right, front, back,
+// bottom, top.  The distance vector dx is one array for all faces, as in the
+// reference: a face sets its own component and keeps those set by earlier
+// faces, so near a corner the later faces act along the combined vector.
+void boundary_eval(const System &s, const Cell &c, Part &pi, double &epot) {
+    static const int order[6] = {0, 1, 2, 3, 4, 5};
+    double dx[3] = {0, 0, 0};
+    for (int face : order) {
+        if (!(c.active & (1u << face))) continue;
+        const int pid = s.wallpot[face * s.max_types + pi.type];
+        if (pid < 0) continue;
+        const Pot &p = s.pots[pid];
+        const int ax = face / 2;
+        const bool low = (face & 1) == 0;
+        const double r = low ? pi.x[ax] : s.h[ax] - pi.x[ax];
+        if (!(r <= p.b)) continue;
+        dx[ax] = low ? r : -r;
+        const Boundary &b = s.bc[face];
+        if (p.kind == KIND_DPD) {
+            dpd_wall(s, p, pi, b.radius, b.vel, dx, r * r);
+            continue;
+        }
+        double e, f;
+        if (!eval_ex(p, pi.radius, b.radius, r * r, e, f)) continue;
+        for (int k = 0; k < 3; k++) pi.f[k] -= f * dx[k];
+        epot += e;
+    }
+}
+
 // runner_doself (tf_runner_dopair.cpp:514-758).
 void do_self(System &s, Cell &c, double &vir) {
     double epot = 0.0;
     const int n = (int)c.parts.size();
     for (int i = 0; i < n; i++) {
         Part &pi = c.parts[i];
+        if (c.active && !s.flux_only) boundary_eval(s, c, pi, epot);
         for (int j = i + 1; j < n; j++) {
             Part &pj = c.parts[j];
             double dx[3];
@@ -459,6 +541,75 @@ void do_pair(System &s, int ci_id, int cj_id, double &vir) {
     ci->epot += epot;
 }
 
+// Cell flags (tfSpace.cpp:784-831) and the task list (tfSpace.cpp:884-999):
+// a neighbour beyond a non-periodic face is skipped instead of wrapped.
+int build_tasks(System &s) {
+    // check_periodicy (tfBoundaryConditions.cpp:192-217), applied to the
+    // faces as set: one periodic face makes both faces of its axis periodic.
+    s.periodic = 0;
+    for (int ax = 0; ax < 3; ax++) {
+        const Boundary &lo = s.bc_set[2 * ax], &hi = s.bc_set[2 * ax + 1];
+        s.bc[2 * ax] = lo;
+        s.bc[2 * ax + 1] = hi;
+        if ((lo.kind & BC_PERIODIC) || (hi.kind & BC_PERIODIC)) {
+            if (!(lo.kind & BC_PERIODIC) || !(hi.kind & BC_PERIODIC)) s.bc[2 * ax].kind = s.bc[2 * ax + 1].kind = BC_PERIODIC;
+            s.periodic |= 1u << ax;
+        }
+    }
+    int l[3];
+    for (Cell &c : s.cells) {
+        c.active = 0;
+        for (int ax = 0; ax < 3; ax++) {
+            if (c.loc[ax] == 0 && (s.bc[2 * ax].kind & BC_ACTIVE))
+                c.active |= 1u << (2 * ax);
+            else if (c.loc[ax] + 1 == s.cdim[ax] && (s.bc[2 * ax + 1].kind & BC_ACTIVE))
+                c.active |= 1u << (2 * ax + 1);
+        }
+    }
+    s.tasks.clear();
+    auto wrap = [&](int v, int ax, bool &ok) {
+        ok = true;
+        if (v < 0 || v >= s.cdim[ax]) {
+            if (!(s.periodic & (1u << ax))) ok = false;
+            v = (v + s.cdim[ax]) % s.cdim[ax];
+        }
+        return v;
+    };
+    for (int i = 0; i < s.cdim[0]; i++)
+        for (int j = 0; j < s.cdim[1]; j++)
+            for (int k = 0; k < s.cdim[2]; k++) {
+                const int id1 = s.cellid(i, j, k);
+                for (l[0] = -s.span[0]; l[0] <= s.span[0]; l[0]++) {
+                    bool okx, oky, okz;
+                    const int ii = wrap(i + l[0], 0, okx);
+                    if (!okx) continue;
+                    for (l[1] = -s.span[1]; l[1] <= s.span[1]; l[1]++) {
+                        const int jj = wrap(j + l[1], 1, oky);
+                        if (!oky) continue;
+                        for (l[2] = -s.span[2]; l[2] <= s.span[2]; l[2]++) {
+                            double d2 = 0;
+                            for (int q = 0; q < 3; q++) {
+                                const double lh = s.h[q] * std::fmax(std::abs(l[q]) - 1, 0);
+                                d2 += lh * lh;
+                            }
+                            if (d2 > s.cutoff2) continue;
+                            const int kk = wrap(k + l[2], 2, okz);
+                            if (!okz) continue;
+                            const int id2 = s.cellid(ii, jj, kk);
+                            if (id1 < id2 || (id1 == id2 && l[0] == 0 && l[1] == 0 && l[2] == 0)) {
+                                int a = id1, b = id2;
+                                double sh[3];
+                                int sid = 0;
+                                if (id1 != id2) sid = getsid(s, a, b, sh);
+                                s.tasks.push_back({id1 == id2 ? 0 : 1, a, b, sid});
+                            }
+                        }
+                    }
+                }
+            }
+    return 0;
+}
+
 int build_space(System &s, const double origin[3], const double dim[3], const int cells[3], double cutoff) {
     s.cutoff = cutoff;
     s.cutoff2 = cutoff * cutoff;
@@ -486,37 +637,7 @@ int build_space(System &s, const double origin[3], const double dim[3], const in
                     c.origin[k] = origin[k] + l[k] * s.h[k];
                 }
             }
-    // Task list (tfSpace.cpp:884-999), periodic in every direction.
-    s.tasks.clear();
-    for (int i = 0; i < s.cdim[0]; i++)
-        for (int j = 0; j < s.cdim[1]; j++)
-            for (int k = 0; k < s.cdim[2]; k++) {
-                const int id1 = s.cellid(i, j, k);
-                for (l[0] = -s.span[0]; l[0] <= s.span[0]; l[0]++) {
-                    const int ii = (i + l[0] + s.cdim[0]) % s.cdim[0];
-                    for (l[1] = -s.span[1]; l[1] <= s.span[1]; l[1]++) {
-                        const int jj = (j + l[1] + s.cdim[1]) % s.cdim[1];
-                        for (l[2] = -s.span[2]; l[2] <= s.span[2]; l[2]++) {
-                            double d2 = 0;
-                            for (int q = 0; q < 3; q++) {
-                                const double lh = s.h[q] * std::fmax(std::abs(l[q]) - 1, 0);
-                                d2 += lh * lh;
-                            }
-                            if (d2 > s.cutoff2) continue;
-                            const int kk = (k + l[2] + s.cdim[2]) % s.cdim[2];
-                            const int id2 = s.cellid(ii, jj, kk);
-                            if (id1 < id2 || (id1 == id2 && l[0] == 0 && l[1] == 0 && l[2] == 0)) {
-                                int a = id1, b = id2;
-                                double sh[3];
-                                int sid = 0;
-                                if (id1 != id2) sid = getsid(s, a, b, sh);
-                                s.tasks.push_back({id1 == id2 ? 0 : 1, a, b, sid});
-                            }
-                        }
-                    }
-                }
-            }
-    return 0;
+    return build_tasks(s);
 }
 
 // Nonbonded pass: sort tasks, then self/pair tasks.  With several threads,
@@ -707,6 +828,40 @@ inline void integrate_species(const System &s, Part &p) {
     }
 }
 
+// apply_update_pos_vel (tf_boundary_eval.h:45-131): a particle past an
+// active free-slip or velocity face of its cell is reflected, faces in the
+// order left, right, front, back, bottom, top; delta is recomputed when a
+// face acted.  Returns true when the particle stays in its cell.
+bool reflect(const System &s, const Cell &c, Part &p, int delta[3]) {
+    const double restitution = 1.0;
+    bool enforced = false;
+    for (int face = 0; face < 6; face++) {
+        if (!(c.active & (1u << face))) continue;
+        const int ax = face / 2;
+        const bool low = (face & 1) == 0;
+        const Boundary &b = s.bc[face];
+        if (low ? !(p.x[ax] <= 0) : !(p.x[ax] >= s.h[ax])) continue;
+        if (b.kind & BC_FREESLIP) {
+            if (low)
+                p.x[ax] = std::max<double>(-p.x[ax] * restitution, 0.0);
+            else
+                p.x[ax] = s.h[ax] - std::max<double>((p.x[ax] - s.h[ax]) * restitution, DBL_EPSILON);
+            p.v[ax] *= -restitution;
+            enforced = true;
+        } else if (b.kind & BC_VELOCITY) {
+            if (low)
+                p.x[ax] = std::max<double>(-p.x[ax] * b.restore, 0.0);
+            else
+                p.x[ax] = s.h[ax] - std::max<double>((p.x[ax] - s.h[ax]) * b.restore, DBL_EPSILON);
+            for (int k = 0; k < 3; k++) p.v[k] = b.vel[k] - ((p.v[k] - b.vel[k]) * b.restore);
+            enforced = true;
+        }
+    }
+    if (enforced)
+        for (int k = 0; k < 3; k++) delta[k] = std::isgreaterequal(p.x[k], s.h[k]) - std::isless(p.x[k], 0.0);
+    return delta[0] == 0 && delta[1] == 0 && delta[2] == 0;
+}
+
 // cell_advance_forward_euler (tf_engine_advance.cpp:140-247) + Fluxes_integrate.
 void advance_cell(System &s, int cid) {
     Cell &c = s.cells[cid];
@@ -744,6 +899,10 @@ void advance_cell(System &s, int cid) {
                 p.x[k] += dx;
                 delta[k] = std::isgreaterequal(p.x[k], s.h[k]) - std::isless(p.x[k], 0.0);
             }
+        }
+        if ((delta[0] != 0 || delta[1] != 0 || delta[2] != 0) && reflect(s, c, p, delta)) {
+            pid++;   // reflected back into this cell
+            continue;
         }
         if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
             for (int k = 0; k < 3; k++) {
@@ -913,6 +1072,7 @@ void step_rk4(System &s, int nthreads) {
                 for (int k = 0; k < 3; k++)
                     if (p.x[k] >= 2.0 * s.h[k] || p.x[k] <= -s.h[k]) s.toofast = 1;
             }
+        if (s.toofast) return;   // CHECK_TOOFAST returns the error from the step
     }
     shuffle(s, false);
 }
@@ -942,6 +1102,7 @@ void *or_create(const double origin[3], const double dim[3], const int cells[3],
     s->max_frac = max_dist_fraction;
     s->max_types = max_types;
     s->potmap.assign((size_t)max_types * max_types, -1);
+    s->wallpot.assign((size_t)6 * max_types, -1);
     s->fluxmap.assign((size_t)max_types * max_types, {});
     s->nspecies = nspecies;
     const bool cubic = s->h[0] == s->h[1] && s->h[1] == s->h[2];
@@ -1083,6 +1244,34 @@ int or_set_dpd_noise(void *h, int mode, unsigned long long seed) {
     return 0;
 }
 
+// One face's boundary condition (BoundaryCondition::init,
+// tfBoundaryConditions.cpp:151-171: NO_SLIP is a velocity face with zero
+// velocity); the cell flags and the task list are rebuilt.
+int or_set_boundary(void *h, int face, unsigned kind, const double vel[3], double restore, double radius) {
+    System *s = static_cast<System *>(h);
+    if (face < 0 || face > 5) return -1;
+    Boundary &b = s->bc_set[face];
+    b.kind = kind;
+    for (int k = 0; k < 3; k++) b.vel[k] = vel ? vel[k] : 0.0;
+    if (kind == BC_NO_SLIP) {
+        b.kind = BC_VELOCITY;
+        b.vel[0] = b.vel[1] = b.vel[2] = 0.0;
+    }
+    b.restore = restore;
+    b.radius = radius;
+    return build_tasks(*s);
+}
+
+// BoundaryCondition::set_potential (tfBoundaryConditions.cpp:504-513);
+// face -1 sets all six (BoundaryConditions::set_potential).
+int or_set_wall_potential(void *h, int face, int type, int pot) {
+    System *s = static_cast<System *>(h);
+    if (face < -1 || face > 5 || type < 0 || type >= s->max_types) return -1;
+    for (int f = 0; f < 6; f++)
+        if (face < 0 || face == f) s->wallpot[f * s->max_types + type] = pot;
+    return 0;
+}
+
 // Universe integrator kind (tfSimulator.cpp:218-229): 0 forward Euler, 1 RK4.
 int or_set_integrator(void *h, int kind) {
     if (kind != 0 && kind != 1) return -1;
@@ -1092,9 +1281,10 @@ int or_set_integrator(void *h, int kind) {
 
 int or_step(void *h, int nsteps, int nthreads) {
     System *s = static_cast<System *>(h);
-    for (int k = 0; k < nsteps; k++) step(*s, nthreads);
-    const int tf = s->toofast.exchange(0);
-    return tf ? -2 : 0;
+    // engine_step returns the error of a particle moving too fast; stepping
+    // stops there (the state is not valid input to another step).
+    for (int k = 0; k < nsteps && !s->toofast; k++) step(*s, nthreads);
+    return s->toofast ? -2 : 0;
 }
 
 // Force evaluation of the current state: kinetic energy, nonbonded and

@@ -35,6 +35,8 @@ _SIG = {
     "or_add_angles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip]),
     "or_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_ulonglong]),
     "or_set_integrator": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "or_set_boundary": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint, _dp, ctypes.c_double, ctypes.c_double]),
+    "or_set_wall_potential": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "or_set_sweep_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "or_get_sweep_mode": (ctypes.c_int, [_vp]),
     "or_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
@@ -107,6 +109,11 @@ class Oracle:
             raise ValueError("oracle rejected the geometry")
         if L.or_set_integrator(self._h, int(scene.get("integrator", 0))) != 0:
             raise ValueError("unknown integrator kind")
+        for b in scene.get("boundaries") or []:
+            vel = np.ascontiguousarray(b.get("velocity", (0.0, 0.0, 0.0)), dtype=np.float64)
+            if L.or_set_boundary(self._h, b["face"], b["kind"], _d(vel), b.get("restore", 1.0),
+                                 b.get("radius", 0.0)) != 0:
+                raise ValueError("oracle rejected a boundary condition")
         for t in scene["types"]:
             L.or_add_type(self._h, t["mass"], t["radius"], t["dynamics"])
         ids = []
@@ -122,6 +129,9 @@ class Oracle:
         self.pot_ids = ids
         for pi, a, b in scene["bind"]:
             L.or_bind(self._h, ids[pi], a, b)
+        for face, t, pi in scene.get("walls") or []:
+            if L.or_set_wall_potential(self._h, face, t, ids[pi]) != 0:
+                raise ValueError("oracle rejected a wall potential")
         for f in scene["fluxes"]:
             L.or_add_flux(self._h, f["kind"], f["ta"], f["tb"], f["sa"], f["sb"], f["k"], f["decay"], f["target"],
                           f["cutoff"])

@@ -106,3 +106,27 @@ def test_integrator_setting(tf):
     assert eng.integrator == tf.RUNGE_KUTTA_4
     e2 = tf.Engine(dim=[10.0] * 3, cells=[3] * 3, cutoff=1.0, integrator=tf.RUNGE_KUTTA_4, nr_fluxsteps=3)
     assert e2.integrator == tf.RUNGE_KUTTA_4
+
+
+def test_boundary_settings(tf):
+    """BoundaryCondition kinds and faces; resetting faces and unknown kinds
+    are rejected; wall potentials need a known type."""
+    eng = tf.Engine(dim=[10.0] * 3, cells=[3] * 3, cutoff=1.0)
+    eng.add_type("A")
+    eng.set_boundary("bottom", tf.BOUNDARY_NO_SLIP)
+    eng.set_boundary("top", tf.BOUNDARY_VELOCITY, velocity=(0.5, 0, 0), restore=0.8)
+    eng.set_boundary(tf.FACE_LEFT, tf.BOUNDARY_POTENTIAL | tf.BOUNDARY_FREESLIP)
+    for bad in (0, 32, 64, tf.BOUNDARY_NO_SLIP | tf.BOUNDARY_FREESLIP):
+        with pytest.raises(tf.TfgError):
+            eng.set_boundary("right", bad)
+    with pytest.raises(tf.TfgError):
+        eng.set_boundary(6, tf.BOUNDARY_FREESLIP)
+    with pytest.raises((tf.TfgError, KeyError)):
+        eng.set_boundary("middle", tf.BOUNDARY_FREESLIP)
+    wall = tf.Potential.harmonic(10.0, 1.0)
+    eng.set_wall_potential("all", 0, wall)
+    eng.set_wall_potential("top", 0, None)
+    with pytest.raises(tf.TfgError):
+        eng.set_wall_potential("top", 3, wall)
+    with pytest.raises(tf.TfgError):
+        eng.set_wall_potential(7, 0, wall)

@@ -29,6 +29,14 @@ PREC_MIXED = 0
 PREC_FP64 = 1
 FORWARD_EULER = 0
 RUNGE_KUTTA_4 = 1
+# boundary condition kinds (bit sets) and faces
+BOUNDARY_VELOCITY = 1
+BOUNDARY_PERIODIC = 2
+BOUNDARY_FREESLIP = 4
+BOUNDARY_POTENTIAL = 8
+BOUNDARY_NO_SLIP = 16
+FACE_LEFT, FACE_RIGHT, FACE_FRONT, FACE_BACK, FACE_BOTTOM, FACE_TOP = range(6)
+FACES = {"left": 0, "right": 1, "front": 2, "back": 3, "bottom": 4, "top": 5}
 
 FROZEN_X = 1 << 3
 FROZEN_Y = 1 << 4
@@ -62,6 +70,9 @@ _SIGNATURES = {
     "tfg_engine_set_nspecies": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tfg_engine_set_integrator": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tfg_engine_integrator": (ctypes.c_int, [_vp]),
+    "tfg_engine_set_boundary": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint32, _dp, ctypes.c_double,
+                                               ctypes.c_double]),
+    "tfg_engine_set_wall_potential": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
     "tfg_engine_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64]),
     "tfg_engine_time": (ctypes.c_int64, [_vp]),
     "tfg_engine_addtype": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
@@ -285,6 +296,20 @@ class Engine:
     @integrator.setter
     def integrator(self, kind):
         _check(lib().tfg_engine_set_integrator(self._h, int(kind)))
+
+    def set_boundary(self, face, kind, velocity=None, restore=1.0, radius=0.0):
+        """One face's boundary condition (face index or name: left, right,
+        front, back, bottom, top; kind a bit set of BOUNDARY_*)."""
+        face = FACES[face] if isinstance(face, str) else int(face)
+        vel = _f64(velocity if velocity is not None else (0.0, 0.0, 0.0), 3)
+        _check(lib().tfg_engine_set_boundary(self._h, face, int(kind), _dptr(vel), float(restore), float(radius)))
+
+    def set_wall_potential(self, face, ptype, pot):
+        """Wall potential of a particle type on a face (-1 or "all": every face)."""
+        face = -1 if face == "all" else (FACES[face] if isinstance(face, str) else int(face))
+        if pot is not None:
+            self._pots.append(pot)
+        _check(lib().tfg_engine_set_wall_potential(self._h, face, int(ptype), pot._h if pot is not None else None))
 
     def set_dpd_noise(self, mode, seed=0):
         _check(lib().tfg_engine_set_dpd_noise(self._h, mode, seed))

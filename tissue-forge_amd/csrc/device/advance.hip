@@ -42,6 +42,127 @@ __device__ inline void eval_table_r(const PotDesc<real> &pd, const real *coef, r
     *eff_out = eff * cc[1];
 }
 
+// potential_eval_ex (tf_potential_eval.h:104-146) for one wall: the radius
+// transform, the clamp at a, false past b; f is force / r.
+template <typename real>
+__device__ inline bool wall_eval_ex(const PotDesc<real> &pd, const real *coef, real ri, real rj, real r2, real *e,
+                                    real *f) {
+    real r = sqrt(r2);
+    const real ro = r < eps_of<real>() ? eps_of<real>() : r;
+    if (pd.flags & POT_SCALED)
+        r = r / (ri + rj);
+    else if (pd.flags & POT_SHIFTED)
+        r = r - (ri + rj) + pd.r0p1;
+    r = r < pd.a ? pd.a : r;
+    if (r > pd.b) return false;
+    real eff;
+    eval_table_r<real>(pd, coef, r, e, &eff);
+    *f = eff / ro;
+    return true;
+}
+
+// boundary_eval (tf_boundary_eval.h:204-254): the wall potentials of the
+// active faces that this particle's cell lies on, faces in the order left,
+// right, front, back, bottom, top.  As in the reference the distance vector
+// is one array across faces: a face sets its own component and keeps those
+// of earlier faces.  DPD walls follow dpd_boundary_eval (tf_dpd_eval.h:78-124)
+// with the constant random term; their energy is 0.  Returns the energy.
+template <typename real>
+__device__ double wall_forces(const KP<real> &P, const SysDesc &sd, int ti, const int loc[3],
+                              const typename V<real>::v4 &p, const typename V<real>::v4 &v,
+                              typename V<real>::v4 *f) {
+    const real pv[3] = {p.x, p.y, p.z};
+    real dx[3] = {0, 0, 0};
+    real fx = 0, fy = 0, fz = 0;
+    double energy = 0;
+    for (int face = 0; face < 6; face++) {
+        const int ax = face >> 1;
+        const bool low = (face & 1) == 0;
+        if (!(sd.bc_kind[face] & BC_ACTIVE)) continue;
+        if (low ? loc[ax] != 0 : loc[ax] != sd.cdim[ax] - 1) continue;
+        const int pid = P.wallmap[sd.wall_off + face * sd.ntypes + ti];
+        if (pid < 0) continue;
+        const PotDesc<real> &pd = P.pots[pid];
+        // distance to the wall in FP64 from the stored coordinate, rounded once
+        const double xd = static_cast<double>(pv[ax]) + sd.foff[ax];
+        const real r = real(low ? xd : sd.h[ax] - xd);
+        if (!(r <= pd.b)) continue;
+        dx[ax] = low ? r : -r;
+        const real r2 = r * r;
+        const real rj = real(sd.bc_radius[face]);
+        if (pd.kind == KIND_DPD) {
+            real rr = sqrt(r2);
+            const real ro = rr < eps_of<real>() ? eps_of<real>() : rr;
+            rr = (pd.flags & POT_SHIFTED) ? rr - (p.w + rj) : rr;
+            if (rr > pd.b) continue;
+            rr = rr >= pd.a ? rr : pd.a;
+            const real e0 = dx[0] / ro, e1 = dx[1] / ro, e2 = dx[2] / ro;
+            const real w0 = v.x - real(sd.bc_vel[face][0]), w1 = v.y - real(sd.bc_vel[face][1]),
+                       w2 = v.z - real(sd.bc_vel[face][2]);
+            const real om = rr < real(0) ? real(1) : (real(1) - rr / pd.b);
+            const real fc = pd.dalpha * om;
+            const real fd = -pd.dgamma * om * om * (e0 * w0 + e1 * w1 + e2 * w2);
+            const real fr = pd.dsigma * om * real(sd.inv_sqrt_dt);
+            const real fm = fc + fd + fr;
+            fx += fm * e0;
+            fy += fm * e1;
+            fz += fm * e2;
+            continue;
+        }
+        real e, fr;
+        if (!wall_eval_ex<real>(pd, P.coef, p.w, rj, r2, &e, &fr)) continue;
+        fx -= fr * dx[0];
+        fy -= fr * dx[1];
+        fz -= fr * dx[2];
+        energy += static_cast<double>(e);
+    }
+    f->x += fx;
+    f->y += fy;
+    f->z += fz;
+    return energy;
+}
+
+// apply_update_pos_vel (tf_boundary_eval.h:45-131): a particle past an
+// active free-slip or velocity face of its cell is reflected (faces in the
+// order left, right, front, back, bottom, top); delta is recomputed when a
+// face acted.  Positions are reflected in the cell frame, in FP64.  On the
+// FP32 path a high-side result that rounds onto the face is moved one ulp
+// inside, so that the particle stays in its cell.
+template <typename real>
+__device__ void reflect(const SysDesc &sd, const int loc[3], real xv[3], real vv[3], int delta[3]) {
+    bool enforced = false;
+    for (int face = 0; face < 6; face++) {
+        const int ax = face >> 1;
+        const bool low = (face & 1) == 0;
+        const int kind = sd.bc_kind[face];
+        if (!(kind & (BC_FREESLIP | BC_VELOCITY))) continue;
+        if (low ? loc[ax] != 0 : loc[ax] != sd.cdim[ax] - 1) continue;
+        const double h = sd.h[ax];
+        double x = static_cast<double>(xv[ax]) + sd.foff[ax];
+        if (low ? !(x <= 0.0) : !(x >= h)) continue;
+        const double rest = (kind & BC_FREESLIP) ? 1.0 : sd.bc_restore[face];
+        x = low ? fmax(-x * rest, 0.0) : h - fmax((x - h) * rest, static_cast<double>(eps_of<real>()));
+        real xn = real(x - sd.foff[ax]);
+        if constexpr (std::is_same<real, float>::value) {
+            const float top = float(h - sd.foff[ax]);
+            if (!low && xn >= top) xn = nextafterf(top, -1e30f);
+        }
+        xv[ax] = xn;
+        if (kind & BC_FREESLIP) {
+            vv[ax] = -vv[ax];
+        } else {
+            for (int k = 0; k < 3; k++) {
+                const real u = real(sd.bc_vel[face][k]);
+                vv[k] = u - ((vv[k] - u) * real(rest));
+            }
+        }
+        enforced = true;
+    }
+    if (enforced)
+        for (int k = 0; k < 3; k++)
+            delta[k] = (xv[k] >= real(sd.h[k] - sd.foff[k])) - (xv[k] < real(-sd.foff[k]));
+}
+
 // RK4 selects the instance with the Runge-Kutta stages (update 2..5); the
 // other serves updates 0 and 1.
 template <typename real, bool RK4>
@@ -147,9 +268,13 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                     f.z += g[2];
                 }
             }
+            const int ti = info & 0xffff;
+            if (sd.bc_active) {
+                const double ew = wall_forces<real>(P, sd, ti, loc, p, v, &f);
+                if (P.energy) P.e_nb[i] += ew;
+            }
             P.force[i] = f;
 
-            const int ti = info & 0xffff;
             const unsigned flags = static_cast<unsigned>(info) >> 16;
             const TypeDesc &td = P.types[sd.type_off + ti];
             if (P.energy) {
@@ -189,6 +314,8 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                         // x + foff >= h, x + foff < 0 in the stored frame
                         delta[k] = (xv[k] >= real(sd.h[k] - sd.foff[k])) - (xv[k] < real(-sd.foff[k]));
                     }
+                    if ((delta[0] != 0 || delta[1] != 0 || delta[2] != 0) && sd.bc_active)
+                        reflect<real>(sd, loc, xv, vv, delta);
                     if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
                         moved = true;
                         for (int k = 0; k < 3; k++) {

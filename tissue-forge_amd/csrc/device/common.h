@@ -48,6 +48,14 @@ struct SysDesc {
     double foff[3];    // stored position = position relative to the cell origin - foff
     double cutoff, cutoff2, dt, dt_flux, w_self, inv_sqrt_dt;
     double maxx[3], maxv[3];
+    // Boundary conditions (faces left, right, front, back, bottom, top):
+    // kinds in effect, wall velocity, restore and wall radius; the wall map
+    // holds 6 * ntypes potential indices (-1: none) from wall_off.
+    int periodic;     // bit k: axis k wraps
+    int bc_active;    // some face is active (free slip, velocity or potential)
+    int wall_off;
+    int bc_kind[6];
+    double bc_vel[6][3], bc_restore[6], bc_radius[6];
 };
 
 template <typename real> struct PotDesc {
@@ -107,6 +115,7 @@ template <typename real> struct KP {
     const TypeDesc *types;
     const FluxDesc<real> *flux;
     const int *fluxmap;   // 2 ints per (system, type pair): first, count (system-relative)
+    const int *wallmap;   // per system, 6 * ntypes wall potential indices
     PArrays<real> A, B;
     typename V<real>::v4 *force;   // force + number density, A order
     int *slot_of;

@@ -71,6 +71,7 @@ struct DeviceEnsemble {
 
     SysDesc *d_sys = nullptr;
     int *d_cell_sys = nullptr, *d_stencil = nullptr, *d_potmap = nullptr, *d_fluxmap = nullptr;
+    int *d_wallmap = nullptr;
     void *d_pots = nullptr, *d_coef = nullptr, *d_flux = nullptr;
     TypeDesc *d_types = nullptr;
     void *pos[2] = {nullptr, nullptr}, *vel[2] = {nullptr, nullptr}, *imass[2] = {nullptr, nullptr};

@@ -29,6 +29,7 @@ template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy) {
     P.types = E->d_types;
     P.flux = static_cast<const FluxDesc<real> *>(E->d_flux);
     P.fluxmap = E->d_fluxmap;
+    P.wallmap = E->d_wallmap;
     PArrays<real> *arr[2] = {&P.A, &P.B};
     for (int s = 0; s < 2; s++) {
         arr[s]->pos = static_cast<v4 *>(E->pos[s]);
@@ -180,7 +181,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     const bool centred = std::is_same<real, float>::value;
 
     std::vector<SysDesc> sys(nsys);
-    std::vector<int> cell_sys, stencil, potmap, fluxmap;
+    std::vector<int> cell_sys, stencil, potmap, fluxmap, wallmap;
     std::vector<PotDesc<real>> pots;
     std::vector<real> coef;
     std::vector<TypeDesc> types;
@@ -307,6 +308,23 @@ template <typename real> static int upload(DeviceEnsemble *E) {
             }
         d.nflux = static_cast<int>(flux.size()) - d.flux_first;
         if (d.nflux > kMaxFluxEntries) return set_error(TFG_E_RANGE, "too many flux entries in one system");
+        // Boundary conditions in effect and the wall potentials.
+        BoundaryFace faces[6];
+        d.periodic = static_cast<int>(e.boundaries(faces));
+        d.bc_active = 0;
+        for (int f = 0; f < 6; f++) {
+            d.bc_kind[f] = static_cast<int>(faces[f].kind);
+            if (faces[f].kind & BC_ACTIVE) d.bc_active = 1;
+            for (int k = 0; k < 3; k++) d.bc_vel[f][k] = faces[f].vel[k];
+            d.bc_restore[f] = faces[f].restore;
+            d.bc_radius[f] = faces[f].radius;
+        }
+        d.wall_off = static_cast<int>(wallmap.size());
+        for (int f = 0; f < 6; f++)
+            for (int a = 0; a < nt; a++) {
+                const int pid = e.wallpot[static_cast<size_t>(f) * e.max_types + a];
+                wallmap.push_back(pid >= 0 && (faces[f].kind & BC_ACTIVE) ? add_pot(pid) : -1);
+            }
         for (int c = 0; c < d.ncells; c++) cell_sys.push_back(s);
         for (const BondRec &b : e.bonds) add_pot(b.pot);
         for (const AngleRec &a : e.angles) add_pot(a.pot);
@@ -395,6 +413,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     ALLOC(E->d_stencil, int, stencil.size());
     ALLOC(E->d_potmap, int, potmap.size());
     ALLOC(E->d_fluxmap, int, fluxmap.size());
+    ALLOC(E->d_wallmap, int, wallmap.size());
     ALLOC(E->d_pots, PotDesc<real>, pots.size());
     ALLOC(E->d_coef, real, coef.size());
     ALLOC(E->d_flux, FluxDesc<real>, flux.size());
@@ -443,6 +462,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     if ((rc = H2D(E->d_stencil, stencil.data(), sizeof(int) * stencil.size()))) return rc;
     if ((rc = H2D(E->d_potmap, potmap.data(), sizeof(int) * potmap.size()))) return rc;
     if ((rc = H2D(E->d_fluxmap, fluxmap.data(), sizeof(int) * fluxmap.size()))) return rc;
+    if ((rc = H2D(E->d_wallmap, wallmap.data(), sizeof(int) * wallmap.size()))) return rc;
     if ((rc = H2D(E->d_pots, pots.data(), sizeof(PotDesc<real>) * pots.size()))) return rc;
     if ((rc = H2D(E->d_coef, coef.data(), sizeof(real) * coef.size()))) return rc;
     if ((rc = H2D(E->d_flux, flux.data(), sizeof(FluxDesc<real>) * flux.size()))) return rc;

@@ -508,14 +508,22 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     const int ox = P.stencil[2 * (sd.stencil_off + col)];
                     const int oy = P.stencil[2 * (sd.stencil_off + col) + 1];
                     int n0 = loc[0] + ox;
+                    int n1 = loc[1] + oy;
+                    // beyond a non-periodic face there is no neighbour
+                    // (space_init's task list skips it, tfSpace.cpp:913-944)
+                    if (!(sd.periodic & 1) && (n0 < 0 || n0 >= sd.cdim[0])) continue;
+                    if (!(sd.periodic & 2) && (n1 < 0 || n1 >= sd.cdim[1])) continue;
                     n0 += n0 < 0 ? sd.cdim[0] : 0;
                     n0 -= n0 >= sd.cdim[0] ? sd.cdim[0] : 0;
-                    int n1 = loc[1] + oy;
                     n1 += n1 < 0 ? sd.cdim[1] : 0;
                     n1 -= n1 >= sd.cdim[1] ? sd.cdim[1] : 0;
                     const int colbase = sd.cell_off + (n0 * sd.cdim[1] + n1) * sd.cdim[2];
                     const double shx = ox * sd.h[0], shy = oy * sd.h[1];
-                    const int zlo = loc[2] - sz, zhi = loc[2] + sz;
+                    int zlo = loc[2] - sz, zhi = loc[2] + sz;
+                    if (!(sd.periodic & 4)) {
+                        zlo = max(zlo, 0);
+                        zhi = min(zhi, sd.cdim[2] - 1);
+                    }
                     // The column's cells are contiguous in memory unless the z range
                     // wraps around the periodic boundary; then it is two runs.
                     const bool wrap_lo = zlo < 0, wrap_hi = zhi >= sd.cdim[2];
@@ -523,7 +531,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     for (int run = 0; run < nruns; run++) {
                         int zf, nzc, oz0;
                         if (!wrap_lo && !wrap_hi) {
-                            zf = zlo; nzc = 2 * sz + 1; oz0 = -sz;
+                            zf = zlo; nzc = zhi - zlo + 1; oz0 = zlo - loc[2];
                         } else if (wrap_lo) {
                             if (run == 0) { zf = zlo + sd.cdim[2]; nzc = -zlo; oz0 = -sz; }
                             else { zf = 0; nzc = zhi + 1; oz0 = -loc[2]; }

@@ -143,6 +143,24 @@ int tfg_engine_integrator(const tfg_engine *h) {
     return h->e.integrator;
 }
 
+int tfg_engine_set_boundary(tfg_engine *h, int face, uint32_t kind, const double velocity[3], double restore,
+                            double radius) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    TFG_GUARD(return with_refresh(h, [&] { return h->e.set_boundary(face, kind, velocity, restore, radius); }));
+}
+
+int tfg_engine_set_wall_potential(tfg_engine *h, int face, int type, const tfg_potential *p) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    TFG_GUARD(return with_refresh(h, [&] {
+        int pot = -1;
+        if (p) {
+            pot = h->e.add_potential(p->p);
+            if (pot < 0) return pot;
+        }
+        return h->e.set_wall_potential(face, type, pot);
+    }));
+}
+
 int tfg_engine_set_nspecies(tfg_engine *h, int nspecies) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
     if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");

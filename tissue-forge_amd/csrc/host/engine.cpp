@@ -66,10 +66,56 @@ int Engine::init(const double origin[3], const double dim[3], const int cells[3]
     if (rc != TFG_OK) return rc;
     max_types = max_types_;
     potmap.assign(static_cast<size_t>(max_types) * max_types, -1);
+    wallpot.assign(static_cast<size_t>(6) * max_types, -1);
     dt = dt_;
     nr_fluxsteps = nr_fluxsteps_;
     dt_flux = dt_ / nr_fluxsteps_;   // tfSimulator.cpp:653
     return TFG_OK;
+}
+
+int Engine::set_boundary(int face, unsigned kind, const double vel[3], double restore, double radius) {
+    if (face < 0 || face > 5) return set_error(TFG_E_INVALIDARG, "face must be in [0, 5]");
+    const unsigned known = BC_VELOCITY | BC_PERIODIC | BC_FREESLIP | BC_POTENTIAL | BC_NO_SLIP | BC_RESETTING;
+    if (kind == 0 || (kind & ~known)) return set_error(TFG_E_INVALIDARG, "invalid boundary condition kind");
+    if (kind & BC_RESETTING)
+        return set_error(TFG_E_INVALIDARG, "resetting boundaries (species reset on crossing) are not supported");
+    if ((kind & BC_NO_SLIP) && kind != BC_NO_SLIP)
+        return set_error(TFG_E_INVALIDARG, "no-slip cannot be combined with other kinds");
+    if (!std::isfinite(restore) || !std::isfinite(radius)) return set_error(TFG_E_INVALIDARG, "non-finite parameter");
+    BoundaryFace b;
+    b.kind = kind;
+    for (int k = 0; k < 3; k++) b.vel[k] = vel ? vel[k] : 0.0;
+    if (kind == BC_NO_SLIP) {   // a velocity face with zero velocity
+        b.kind = BC_VELOCITY;
+        b.vel[0] = b.vel[1] = b.vel[2] = 0.0;
+    }
+    b.restore = restore;
+    b.radius = radius;
+    bc_set[face] = b;
+    return TFG_OK;
+}
+
+int Engine::set_wall_potential(int face, int type, int pot) {
+    if (face < -1 || face > 5) return set_error(TFG_E_INVALIDARG, "face must be in [-1, 5]");
+    if (type < 0 || type >= static_cast<int>(types.size())) return set_error(TFG_E_INVALIDARG, "unknown particle type");
+    if (pot < -1 || pot >= static_cast<int>(pots.size())) return set_error(TFG_E_INVALIDARG, "unknown potential");
+    for (int f = 0; f < 6; f++)
+        if (face < 0 || face == f) wallpot[static_cast<size_t>(f) * max_types + type] = pot;
+    return TFG_OK;
+}
+
+unsigned Engine::boundaries(BoundaryFace out[6]) const {
+    unsigned periodic = 0;
+    for (int ax = 0; ax < 3; ax++) {
+        out[2 * ax] = bc_set[2 * ax];
+        out[2 * ax + 1] = bc_set[2 * ax + 1];
+        const bool lo = bc_set[2 * ax].kind & BC_PERIODIC, hi = bc_set[2 * ax + 1].kind & BC_PERIODIC;
+        if (lo || hi) {
+            if (!(lo && hi)) out[2 * ax].kind = out[2 * ax + 1].kind = BC_PERIODIC;
+            periodic |= 1u << ax;
+        }
+    }
+    return periodic;
 }
 
 int Engine::add_type(const std::string &name, double mass, double radius, int dynamics) {

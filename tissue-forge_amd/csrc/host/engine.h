@@ -35,6 +35,29 @@ enum DpdNoise : int { DPD_NOISE_UNIT = 0, DPD_NOISE_PHILOX = 1 };
 // (tfSimulator.cpp:218-229).
 enum Integrator : int { INTEG_FORWARD_EULER = 0, INTEG_RUNGE_KUTTA_4 = 1 };
 
+// BoundaryConditionKind (tfBoundaryConditions.h:38-46).
+enum BoundaryKind : unsigned {
+    BC_VELOCITY = 1u << 0,
+    BC_PERIODIC = 1u << 1,
+    BC_FREESLIP = 1u << 2,
+    BC_POTENTIAL = 1u << 3,
+    BC_NO_SLIP = 1u << 4,
+    BC_RESETTING = 1u << 5,
+    BC_ACTIVE = BC_FREESLIP | BC_VELOCITY | BC_POTENTIAL,
+};
+// Faces, low and high side of each axis: left/right (x), front/back (y),
+// bottom/top (z).
+enum Face : int { FACE_LEFT = 0, FACE_RIGHT = 1, FACE_FRONT = 2, FACE_BACK = 3, FACE_BOTTOM = 4, FACE_TOP = 5 };
+
+// One face's BoundaryCondition (tfBoundaryConditions.h:48-78): kind, wall
+// velocity, restore coefficient and the wall's particle radius.
+struct BoundaryFace {
+    unsigned kind = BC_PERIODIC;
+    double vel[3] = {0, 0, 0};
+    double restore = 1.0;
+    double radius = 0.0;
+};
+
 struct ParticleType {
     std::string name;
     double mass = 1.0, imass = 1.0, radius = 1.0;
@@ -105,6 +128,8 @@ class Engine {
     double dt_flux = 0.01;
     double max_dist_fraction = 0.05;   // tfEngine.cpp:1865
     int integrator = INTEG_FORWARD_EULER;
+    BoundaryFace bc_set[6];        // faces as set (every face periodic by default)
+    std::vector<int> wallpot;      // 6 * max_types: pot id of (face, type) or -1
     int64_t time = 0;                  // step counter (engine->time)
     int dpd_noise = DPD_NOISE_UNIT;
     uint64_t dpd_seed = 0;
@@ -122,6 +147,14 @@ class Engine {
     int bind(int pot, int ti, int tj);
     int add_flux(int kind, int ta, int tb, int ia, int ib, double k, double decay,
                  double target, double cutoff);
+    // BoundaryCondition::init (tfBoundaryConditions.cpp:151-171).
+    int set_boundary(int face, unsigned kind, const double vel[3], double restore, double radius);
+    // BoundaryCondition::set_potential; face -1: all six faces.
+    int set_wall_potential(int face, int type, int pot);
+    // The faces in effect: check_periodicy (tfBoundaryConditions.cpp:192-217)
+    // makes both faces of an axis periodic when one is.  Returns the
+    // periodic axes as bits.
+    unsigned boundaries(BoundaryFace out[6]) const;
     int add_particles(int n, const int *types, const double *x, const double *v,
                       const double *sp, int32_t *ids_out);
     int add_bonds(int pot, int n, const int32_t *ij);

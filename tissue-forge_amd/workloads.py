@@ -20,12 +20,21 @@ Workloads (particle counts are the full sizes; every builder takes a scale):
                   like types, LJ between unlike types, cutoff 2.5.
   ensemble        512 cell-sorting replicas, N log-uniform in [500, 20000],
                   density 1, d_AB uniform in [0.2, 2.0].
+  dpd_couette     DPD fluid (the dpd_droplets parameters, one type) between a
+                  no-slip bottom wall and a top wall moving at (1, 0, 0),
+                  periodic in x and y (after examples/py/dpd2.py of the
+                  reference); optionally DPD wall potentials on both walls.
+  walled_morse    the cell-sorting mixture in a closed box: every face a
+                  potential + free-slip wall, Morse walls for both types and
+                  a harmonic top wall for type B (after
+                  examples/py/boundary_morse.py).
 """
 import math
 
 import numpy as np
 
-from . import (DPD_NOISE_PHILOX, FLUX_FICK, NEWTONIAN, OVERDAMPED, Engine, Potential)
+from . import (BOUNDARY_FREESLIP, BOUNDARY_NO_SLIP, BOUNDARY_POTENTIAL, BOUNDARY_VELOCITY, DPD_NOISE_PHILOX,
+               FLUX_FICK, NEWTONIAN, OVERDAMPED, Engine, Potential)
 
 
 def _cells_for(L, rc):
@@ -118,6 +127,34 @@ def bonded_chains(nchains=31_250, beads=32, density=0.8, seed=3, dt=0.002, cutof
                 angles=dict(pot=2, ijk=ijk), dpd_noise=None)
 
 
+def dpd_couette(n=24_000, density=3.0, seed=6, dt=0.01, wall_speed=1.0, noise=None, noise_seed=99, wall_dpd=False):
+    Lx = Ly = (n / density / 0.5) ** (1.0 / 3.0)
+    Lz = 0.5 * Lx
+    cutoff = 1.0
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, 1.0, size=(n, 3)) * [Lx, Ly, Lz]
+    pots = [dict(kind="dpd", alpha=25.0, gamma=4.5, sigma=3.0, cutoff=cutoff, shifted=False)]
+    cells = [_cells_for(Lx, cutoff), _cells_for(Ly, cutoff), _cells_for(Lz, cutoff)]
+    bcs = [dict(face=4, kind=BOUNDARY_NO_SLIP), dict(face=5, kind=BOUNDARY_VELOCITY, velocity=(wall_speed, 0.0, 0.0))]
+    return dict(name="dpd_couette", dim=[Lx, Ly, Lz], cells=cells, cutoff=cutoff, dt=dt, nspecies=0,
+                types=[dict(name="fluid", mass=1.0, radius=0.5, dynamics=NEWTONIAN)], pots=pots,
+                bind=[(0, 0, 0)], fluxes=[], x=x, v=np.zeros((n, 3)), ptype=np.zeros(n, dtype=np.int32),
+                species=None, bonds=None, angles=None, boundaries=bcs,
+                walls=[(4, 0, 0), (5, 0, 0)] if wall_dpd else [],
+                dpd_noise=None if noise is None else (DPD_NOISE_PHILOX if noise == "philox" else 0, noise_seed))
+
+
+def walled_morse(n=8000, L=20.0, cells=4, seed=7, dt=0.005, dynamics=OVERDAMPED):
+    sc = cell_sorting(n=n, L=L, cells=cells, seed=seed, dt=dt)
+    sc["name"] = "walled_morse"
+    sc["types"] = [dict(t, dynamics=dynamics) for t in sc["types"]]
+    sc["pots"] = sc["pots"] + [dict(kind="morse", d=1.0, a=5.0, r0=0.6, min=0.05, max=2.0, tol=1e-4, shifted=False),
+                               dict(kind="harmonic", k=20.0, r0=1.5, min=0.0, max=2.0, tol=1e-6)]
+    sc["boundaries"] = [dict(face=f, kind=BOUNDARY_POTENTIAL | BOUNDARY_FREESLIP) for f in range(6)]
+    sc["walls"] = [(-1, 0, 3), (-1, 1, 3), (5, 1, 4)]
+    return sc
+
+
 def ensemble(nrep=512, seed=5, nmin=500, nmax=20000, dt=0.005):
     rng = np.random.default_rng(seed)
     ns = np.rint(np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=nrep))).astype(int)
@@ -164,6 +201,10 @@ def build_engine(scene, potentials=None, max_distance_fraction=None):
         eng.bind(pots[pi], a, b)
     for f in scene["fluxes"]:
         eng.add_flux(f["kind"], f["ta"], f["tb"], f["sa"], f["sb"], f["k"], f["decay"], f["target"], f["cutoff"])
+    for b in scene.get("boundaries") or []:
+        eng.set_boundary(b["face"], b["kind"], b.get("velocity"), b.get("restore", 1.0), b.get("radius", 0.0))
+    for face, t, pi in scene.get("walls") or []:
+        eng.set_wall_potential(face, t, pots[pi])
     if scene.get("dpd_noise"):
         eng.set_dpd_noise(*scene["dpd_noise"])
     eng.add_particles(scene["ptype"], scene["x"], scene["v"], scene["species"])
