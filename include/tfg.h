@@ -99,6 +99,24 @@ int tfg_engine_set_boundary(tfg_engine *e, int face, uint32_t kind, const double
  * a particle type on one face, or on all six with face = -1; NULL clears it.
  * Evaluated by boundary_eval (source/mdcore/src/tf_boundary_eval.h:204-254). */
 int tfg_engine_set_wall_potential(tfg_engine *e, int face, int type, const tfg_potential *p);
+/* Neighbour lists with a skin: the role of engine_flag_verlet /
+ * engine_verlet_update (source/mdcore/src/tfEngine.cpp:214-354,
+ * tf_runner_verlet.cpp).  Lists hold every pair within cutoff + skin at their
+ * build and are rebuilt, on the device, once a particle has moved more than
+ * skin / 2 since (the reference's 2 maxdx > skin, tfEngine.cpp:287-288).  The
+ * pair set of every step is the one engine_step uses, r^2 <= cutoff^2, and
+ * the integrator is unchanged (the reference's Verlet mode also swaps the
+ * integrator; that is not reproduced).  skin = 0 turns lists off; capacity
+ * is the entries per particle (0: from the density); a step that would
+ * exceed it fails with TFG_E_RANGE.  Lists are used when every engine of the
+ * device copy asks for them, integrates with forward Euler, has tabulated
+ * potentials only (no DPD, flux or radius transforms) whose tables fit the
+ * LDS cache, and its cutoff + skin spans at most two cells per axis
+ * (periodic axes of five cells or more); otherwise the cell pass runs. */
+int tfg_engine_set_verlet(tfg_engine *e, double skin, int capacity);
+/* Whether the device copy steps with lists, the rebuilds so far and the
+ * capacity; zeros when the engine is not on the device. */
+int tfg_engine_verlet_state(const tfg_engine *e, int *active, int64_t *rebuilds, int *capacity);
 /* Species count per particle (the state vector of source/state/, flat here). */
 int tfg_engine_set_nspecies(tfg_engine *e, int nspecies);
 int tfg_engine_set_dpd_noise(tfg_engine *e, int mode, uint64_t seed);

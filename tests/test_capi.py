@@ -130,3 +130,14 @@ def test_boundary_settings(tf):
         eng.set_wall_potential("top", 3, wall)
     with pytest.raises(tf.TfgError):
         eng.set_wall_potential(7, 0, wall)
+
+
+def test_verlet_settings(tf):
+    eng = tf.Engine(dim=[10.0] * 3, cells=[3] * 3, cutoff=1.0)
+    assert eng.verlet_state() == (False, 0, 0)   # not on the device
+    eng.set_verlet(0.3)
+    eng.set_verlet(0.3, capacity=128)
+    eng.set_verlet(0.0)
+    for skin, cap in ((-0.1, 0), (float("nan"), 0), (0.3, -1)):
+        with pytest.raises(tf.TfgError):
+            eng.set_verlet(skin, cap)

@@ -73,6 +73,9 @@ _SIGNATURES = {
     "tfg_engine_set_boundary": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint32, _dp, ctypes.c_double,
                                                ctypes.c_double]),
     "tfg_engine_set_wall_potential": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "tfg_engine_set_verlet": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int]),
+    "tfg_engine_verlet_state": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.POINTER(ctypes.c_int)]),
     "tfg_engine_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64]),
     "tfg_engine_time": (ctypes.c_int64, [_vp]),
     "tfg_engine_addtype": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
@@ -310,6 +313,19 @@ class Engine:
         if pot is not None:
             self._pots.append(pot)
         _check(lib().tfg_engine_set_wall_potential(self._h, face, int(ptype), pot._h if pot is not None else None))
+
+    def set_verlet(self, skin, capacity=0):
+        """Neighbour lists with a skin (0 turns them off); the pair set of
+        every step is unchanged."""
+        _check(lib().tfg_engine_set_verlet(self._h, float(skin), int(capacity)))
+
+    def verlet_state(self):
+        """(active, rebuilds, capacity) of the device copy."""
+        a = ctypes.c_int()
+        r = ctypes.c_int64()
+        c = ctypes.c_int()
+        _check(lib().tfg_engine_verlet_state(self._h, ctypes.byref(a), ctypes.byref(r), ctypes.byref(c)))
+        return bool(a.value), r.value, c.value
 
     def set_dpd_noise(self, mode, seed=0):
         _check(lib().tfg_engine_set_dpd_noise(self._h, mode, seed))

@@ -176,6 +176,7 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
         const int i = base + lane;
         const bool valid = i < P.np;
         int newc = -1;
+        bool far_moved = false;
         if (valid) {
             const int4 mi = P.A.meta[i];
             const int c = mi.z;
@@ -431,6 +432,32 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
             P.B.vel[i] = v;
             P.B.meta[i] = make_int4(info, id, newc, 0);
             P.B.imass[i] = P.A.imass[i];
+            if (!RK4 && P.list_mode && update == 1) {
+                // Position in the frame of the build cell (one FP64 rounding)
+                // and the displacement since the build: beyond skin / 2 the
+                // next step rebuilds the lists.
+                int l0[3];
+                cell_loc(P.cid0[i] - sd.cell_off, sd, l0);
+                const real pv[3] = {p.x, p.y, p.z};
+                real xbv[3];
+                for (int k = 0; k < 3; k++) {
+                    int d = loc[k] - l0[k];
+                    if (sd.periodic & (1 << k)) {
+                        if (d > 1) d -= sd.cdim[k];
+                        else if (d < -1) d += sd.cdim[k];
+                    }
+                    xbv[k] = real(static_cast<double>(pv[k]) + d * sd.h[k]);
+                }
+                const v4 o = P.x0[i];
+                const real ex = xbv[0] - o.x, ey = xbv[1] - o.y, ez = xbv[2] - o.z;
+                far_moved = ex * ex + ey * ey + ez * ez > real(sd.skin_half2);
+                v4 b;
+                b.x = xbv[0];
+                b.y = xbv[1];
+                b.z = xbv[2];
+                b.w = as_real_bits((info & 0xffff) * kPairVecs * static_cast<int>(sizeof(v4)), real(0));
+                P.xb[i] = b;
+            }
             // Particles that changed cell skip this step's integration and keep
             // their pending flux (Fluxes_integrate runs over the particles left
             // in the cell, tf_engine_advance.cpp:430-436).
@@ -445,6 +472,19 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                 P.B.sp[o] = s;
                 P.B.q[o] = qq;
             }
+        }
+        if (P.list_mode) {
+            // One request per system present in the wave, and none once its
+            // flag is up.
+            const int sysid = valid ? P.cell_sys[newc] : -1;
+            unsigned long long req = __ballot(far_moved);
+            while (req != 0ull) {
+                const int leader = __builtin_ctzll(req);
+                const int s = __shfl(sysid, leader);
+                req &= ~__ballot(sysid == s);
+                if (lane == leader && P.sflags[s] == 0) atomicOr(&P.sflags[s], 1);
+            }
+            continue;   // the counts are taken afresh when the lists are rebuilt
         }
         int head, len;
         wave_runs(newc, valid, lane, &head, &len);

@@ -9,6 +9,12 @@
 //   permute   gather of every particle array from buffer B into buffer A
 // The cell id of every particle is computed by the advance pass with the
 // reference's incremental rule; binning only reorders.
+//
+// With neighbour lists the particle order is kept between rebuilds: every
+// kernel here takes the step's rebuild flag (gate) and returns at once
+// without it, except the permute, which then copies buffer B to A in place
+// of the gather.  The cell counts are taken afresh (k_count) in the steps
+// that rebuild.
 #include <algorithm>
 
 #include "ensemble.h"
@@ -43,7 +49,8 @@ __device__ inline int block_exclusive_scan(int v, int *s) {
     return before + x - v;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int *cnt, int *bsum, int n) {
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int *cnt, int *bsum, int n, const int *gate) {
+    if (gate && *gate == 0) return;
     __shared__ int s[16];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
     int v = 0;
@@ -54,7 +61,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int *cnt, int 
 }
 
 // Exclusive scan of the tile totals in place; bsum[nb] = grand total.
-__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(int *bsum, int nb) {
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(int *bsum, int nb, const int *gate) {
+    if (gate && *gate == 0) return;
     __shared__ int s[16];
     int carry = 0;
     for (int b0 = 0; b0 < nb; b0 += kScanBlock) {
@@ -71,7 +79,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_tiles(int *bsum, int nb) {
     if (threadIdx.x == 0) bsum[nb] = carry;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int *cnt, const int *bsum, int *start, int n, int nb) {
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int *cnt, const int *bsum, int *start, int n, int nb,
+                                                          const int *gate) {
+    if (gate && *gate == 0) return;
     __shared__ int s[16];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
     int c[kScanPer];
@@ -89,7 +99,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int *cnt, const
 }
 
 // Counts per cell from buffer B (used once after an upload).
-__global__ void k_count(const int4 *meta, int *cnt, int np) {
+__global__ void k_count(const int4 *meta, int *cnt, int np, const int *gate) {
+    if (gate && *gate == 0) return;
     const int lane = threadIdx.x & 63;
     const int wstart = (blockIdx.x * blockDim.x + threadIdx.x) - lane;
     const int stride = gridDim.x * blockDim.x;
@@ -105,7 +116,8 @@ __global__ void k_count(const int4 *meta, int *cnt, int np) {
 
 // Places particle i of buffer B at start[cell] + (reserved offset); one
 // atomic per run of equal cell ids in the wave.
-__global__ void k_scatter(const int4 *meta, const int *start, int *fill, int2 *tmp, int np) {
+__global__ void k_scatter(const int4 *meta, const int *start, int *fill, int2 *tmp, int np, const int *gate) {
+    if (gate && *gate == 0) return;
     const int lane = threadIdx.x & 63;
     const int wstart = (blockIdx.x * blockDim.x + threadIdx.x) - lane;
     const int stride = gridDim.x * blockDim.x;
@@ -129,7 +141,8 @@ __global__ void k_scatter(const int4 *meta, const int *start, int *fill, int2 *t
 // 16-lane groups with shuffles, four cells per wave; larger cells use the
 // whole wave.
 __global__ __launch_bounds__(64) void k_cellorder(const int *start, const int2 *tmp, int *perm, int *cnt,
-                                                  int *fill, int ncells) {
+                                                  int *fill, int ncells, const int *gate) {
+    if (gate && *gate == 0) return;
     __shared__ int keys[2048];
     const int lane = threadIdx.x;
     const int g = lane >> 4, e = lane & 15;
@@ -176,9 +189,15 @@ __global__ __launch_bounds__(64) void k_cellorder(const int *start, const int2 *
 
 template <typename real>
 __global__ void k_permute(KP<real> P) {
+    using v4 = typename V<real>::v4;
+    constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
     const int S = P.S;
+    const bool any = !P.list_mode || P.lflags[0] != 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.np; i += gridDim.x * blockDim.x) {
-        const int src = P.perm[i];
+        // slot i belongs to the system of the particle buffer B holds there;
+        // systems that keep their lists keep their order
+        const bool rebuild = any && (!P.list_mode || P.sflags[P.nsys + P.cell_sys[P.B.meta[i].z]] != 0);
+        const int src = rebuild ? P.perm[i] : i;
         P.A.pos[i] = P.B.pos[src];
         P.A.vel[i] = P.B.vel[src];
         const int4 m = P.B.meta[src];
@@ -187,6 +206,13 @@ __global__ void k_permute(KP<real> P) {
         for (int k = 0; k < S; k++) {
             P.A.sp[(size_t)k * P.np + i] = P.B.sp[(size_t)k * P.np + src];
             P.A.q[(size_t)k * P.np + i] = P.B.q[(size_t)k * P.np + src];
+        }
+        if (P.list_mode && rebuild) {   // the state the new lists are built from
+            v4 p = P.A.pos[i];
+            P.x0[i] = p;
+            P.cid0[i] = m.z;
+            p.w = as_real_bits((m.x & 0xffff) * kPairBytes, real(0));
+            P.xb[i] = p;
         }
         if (P.A.p0) {   // Runge-Kutta stage state
             P.A.p0[i] = P.B.p0[src];
@@ -201,20 +227,24 @@ __global__ void k_permute(KP<real> P) {
 
 int run_count(DeviceEnsemble *E) {
     hipLaunchKernelGGL(k_count, dim3(E->stream_grid()), dim3(E->shape.stream_threads), 0, E->stream, E->meta[1],
-                       E->cnt, E->np);
+                       E->cnt, E->np, nullptr);
     return launch_error("count");
 }
 
 template <typename real> int run_bin(DeviceEnsemble *E, const KP<real> &P) {
     hipStream_t s = E->stream;
     const int nb = (E->ncells + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->ncells);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanBlock), 0, s, E->bsum, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->start, E->ncells, nb);
     const int g = E->stream_grid(), t = E->shape.stream_threads;
-    hipLaunchKernelGGL(k_scatter, dim3(g), dim3(t), 0, s, E->meta[1], E->start, E->fill, E->tmp, E->np);
+    const int *gate = E->list_mode ? E->d_lflags : nullptr;
+    if (E->list_mode)
+        hipLaunchKernelGGL(k_count, dim3(g), dim3(t), 0, s, E->meta[1], E->cnt, E->np, gate);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->ncells, gate);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanBlock), 0, s, E->bsum, nb, gate);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->start, E->ncells, nb, gate);
+    hipLaunchKernelGGL(k_scatter, dim3(g), dim3(t), 0, s, E->meta[1], E->start, E->fill, E->tmp, E->np, gate);
     const int oc = std::min((E->ncells + 3) / 4, 65536);
-    hipLaunchKernelGGL(k_cellorder, dim3(oc), dim3(64), 0, s, E->start, E->tmp, E->perm, E->cnt, E->fill, E->ncells);
+    hipLaunchKernelGGL(k_cellorder, dim3(oc), dim3(64), 0, s, E->start, E->tmp, E->perm, E->cnt, E->fill, E->ncells,
+                       gate);
     hipLaunchKernelGGL(k_permute<real>, dim3(g), dim3(t), 0, s, P);
     return launch_error("binning");
 }

@@ -56,7 +56,20 @@ struct SysDesc {
     int wall_off;
     int bc_kind[6];
     double bc_vel[6][3], bc_restore[6], bc_radius[6];
+    // Neighbour lists: the build stencil covers cutoff + skin; a particle
+    // displaced by more than skin / 2 since the build requests a rebuild.
+    int nstencil_b, stencil_off_b, span_z_b;
+    double cutb2;        // (cutoff + skin)^2, widened by a relative 1e-5
+    double skin_half2;   // (skin / 2)^2, narrowed by a relative 1e-4
 };
+
+// Neighbour-list entry: the partner's slot, and its build cell's offset from
+// the particle's build cell as ((ox + 2) * 5 + (oy + 2)) * 5 + (oz + 2).
+constexpr int kListSlotBits = 25;
+constexpr int kListSlotMask = (1 << kListSlotBits) - 1;
+__host__ __device__ inline int list_code(int ox, int oy, int oz) { return ((ox + 2) * 5 + (oy + 2)) * 5 + (oz + 2); }
+// Pair descriptors: vectors per type pair (the pair kernels' LDS table).
+constexpr int kPairVecs = 4;
 
 template <typename real> struct PotDesc {
     int kind;
@@ -131,6 +144,18 @@ template <typename real> struct KP {
     int step_ahead;   // 1 when forces are evaluated outside a step (time + 1)
     int coef_lds;     // 1 when every system's pair tables fit the LDS cache
     NbLayout nb;
+    // Neighbour lists (null and 0 without them).
+    int list_mode;
+    int lcap;                         // entries per particle
+    int *list;                        // entry k of particle i at list[k * np + i]
+    int *lcount;                      // entries per particle at the build
+    const int4 *lblock;               // list kernel blocks: system, first slot, count
+    typename V<real>::v4 *xb;         // position in the frame of the build cell; w: type column bits
+    typename V<real>::v4 *x0;         // stored position at the build
+    int *cid0;                        // cell at the build
+    int *lflags;                      // [0] some system rebuilds in this step, [2] overflow
+    int *sflags;                      // per system: [s] requested, [nsys + s] rebuilds now, [2 nsys + s] rebuilds done
+    int nsys;
 };
 
 __device__ inline float as_real_bits(int v, float) { return __int_as_float(v); }

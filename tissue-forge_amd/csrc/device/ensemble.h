@@ -55,6 +55,14 @@ struct DeviceEnsemble {
     int nr_fluxsteps = 1;                   // shared as well
     int coef_lds_bytes = 0;   // table bytes cached in LDS by the pair kernel (0: read from global)
     NbLayout nb{};            // LDS layout of the pair kernel
+    NbLayout nbb{};           // LDS layout of the list-build variant
+    NbLayout nbl{};           // LDS of the list pair kernel (off_coef: tables, off_waves: cell offsets)
+    bool list_mode = false;   // neighbour lists in use (every member asked for them and can use them)
+    int lcap = 0;
+    int nlblocks = 0;
+    int4 *d_lblock = nullptr;
+    int *d_list = nullptr, *d_lcount = nullptr, *d_cid0 = nullptr, *d_lflags = nullptr, *d_sflags = nullptr;
+    void *xb = nullptr, *x0 = nullptr;
     int nonbond_resident = 0; // pair-kernel workgroups resident on the device at once
     LaunchShape shape;
     Buffers buf;
@@ -125,6 +133,10 @@ template <typename real> int run_nonbond(DeviceEnsemble *E, const KP<real> &P);
 template <typename real> int nonbond_setup(DeviceEnsemble *E, int max_types, int max_coef);
 // update: 0 forces only, 1 forward-Euler update, 2..5 Runge-Kutta stage 1..4.
 template <typename real> int run_advance(DeviceEnsemble *E, const KP<real> &P, int update);
+// Neighbour lists: the build (a variant of the pair kernel over the wider
+// stencil, run when the step rebuilds) and the list pair kernel.
+template <typename real> int run_list_build(DeviceEnsemble *E, const KP<real> &P);
+template <typename real> int run_list_force(DeviceEnsemble *E, const KP<real> &P);
 // Fluxes_integrate over every particle of buffer A (the flux sub-steps).
 template <typename real> int run_flux_integrate(DeviceEnsemble *E, const KP<real> &P);
 template <typename real> int run_energy(DeviceEnsemble *E, const KP<real> &P);

@@ -71,11 +71,55 @@ template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy) {
     P.step_ahead = 0;
     P.coef_lds = E->coef_lds_bytes > 0 ? 1 : 0;
     P.nb = E->nb;
+    P.list_mode = E->list_mode ? 1 : 0;
+    P.lcap = E->lcap;
+    P.list = E->d_list;
+    P.lcount = E->d_lcount;
+    P.lblock = E->d_lblock;
+    P.xb = static_cast<v4 *>(E->xb);
+    P.x0 = static_cast<v4 *>(E->x0);
+    P.cid0 = E->d_cid0;
+    P.lflags = E->d_lflags;
+    P.sflags = E->d_sflags;
+    P.nsys = E->nsys;
     return P;
 }
 
 __global__ void k_tick(long long *step) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+// List mode: a system rebuilds in this step when the last advance asked for
+// it (or its state was just uploaded or rebinned); the requests are
+// consumed.  lflags[0] tells the binning kernels that some system rebuilds.
+__global__ __launch_bounds__(256) void k_tick_list(long long *step, int *lflags, int *sflags, int nsys) {
+    __shared__ int any;
+    if (threadIdx.x == 0) {
+        *step += 1;
+        any = 0;
+    }
+    __syncthreads();
+    int mine = 0;
+    for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
+        const int rb = sflags[s] != 0 ? 1 : 0;
+        sflags[s] = 0;
+        sflags[nsys + s] = rb;
+        sflags[2 * nsys + s] += rb;
+        mine |= rb;
+    }
+    if (mine) atomicOr(&any, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) lflags[0] = any;
+}
+
+// A force pass outside a step rebins every system and leaves the next step
+// a rebuild to do, since the order changed under the lists.
+__global__ __launch_bounds__(256) void k_list_rebin(int *lflags, int *sflags, int nsys) {
+    if (threadIdx.x == 0) lflags[0] = 1;
+    for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
+        sflags[s] = 1;
+        sflags[nsys + s] = 1;
+    }
 }
 
 // One step (engine_step): tick, then
@@ -90,6 +134,21 @@ template <typename real> static int enqueue_step(DeviceEnsemble *E, bool energy,
     const KP<real> P = make_params<real>(E, 0);
     const KP<real> Pe = make_params<real>(E, energy ? 1 : 0);
     int rc;
+    if (E->list_mode) {
+        // Neighbour lists: bin (or copy) B into A, rebuild the lists when the
+        // flag is up, list pairs, advance (which raises the flag).
+        hipLaunchKernelGGL(k_tick_list, dim3(1), dim3(256), 0, E->stream, E->d_step, E->d_lflags, E->d_sflags, E->nsys);
+        if (ev) TFG_HIP(hipEventRecord(ev[0], E->stream));
+        if ((rc = run_bin<real>(E, Pe)) != TFG_OK) return rc;
+        if (ev) TFG_HIP(hipEventRecord(ev[1], E->stream));
+        if ((rc = run_list_build<real>(E, Pe)) != TFG_OK) return rc;
+        if ((rc = run_list_force<real>(E, Pe)) != TFG_OK) return rc;
+        if (ev) TFG_HIP(hipEventRecord(ev[2], E->stream));
+        if ((rc = run_advance<real>(E, Pe, 1)) != TFG_OK) return rc;
+        if (ev) TFG_HIP(hipEventRecord(ev[3], E->stream));
+        if (energy && (rc = run_energy<real>(E, Pe)) != TFG_OK) return rc;
+        return TFG_OK;
+    }
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, E->stream, E->d_step);
     const bool rk4 = E->integrator == INTEG_RUNGE_KUTTA_4;
     const int rounds = rk4 ? 4 : 1;
@@ -193,6 +252,8 @@ template <typename real> static int upload(DeviceEnsemble *E) {
 
     // Registry id -> device potential index, per system.
     std::vector<std::vector<int>> remaps(nsys);
+    bool list_ok = E->integrator == INTEG_FORWARD_EULER;   // neighbour lists for every member
+    int lcap = 0;
     int np = 0, nc = 0, max_coef = 0;
     for (int s = 0; s < nsys; s++) {
         Engine &e = *E->engines[s];
@@ -230,6 +291,40 @@ template <typename real> static int upload(DeviceEnsemble *E) {
                 stencil.push_back(oy);
             }
         d.nstencil = static_cast<int>(stencil.size() / 2) - d.stencil_off;
+        // Neighbour lists: build stencil for cutoff + skin (at most two cells
+        // per axis, five cells or more along periodic axes), the build radius
+        // widened and the rebuild threshold narrowed by small margins.
+        if (e.verlet_skin > 0) {
+            const double rcs = e.geom.cutoff + e.verlet_skin;
+            int sb[3];
+            BoundaryFace fb[6];
+            const unsigned per = e.boundaries(fb);
+            for (int k = 0; k < 3; k++) {
+                sb[k] = static_cast<int>(std::ceil(rcs / e.geom.h[k]));
+                if (sb[k] > 2 || ((per >> k & 1u) && e.geom.cdim[k] < 2 * sb[k] + 1)) list_ok = false;
+            }
+            d.span_z_b = sb[2];
+            d.stencil_off_b = static_cast<int>(stencil.size() / 2);
+            for (int ox = -sb[0]; ox <= sb[0]; ox++)
+                for (int oy = -sb[1]; oy <= sb[1]; oy++) {
+                    const double lx = e.geom.h[0] * std::fmax(std::abs(ox) - 1, 0);
+                    const double ly = e.geom.h[1] * std::fmax(std::abs(oy) - 1, 0);
+                    if (lx * lx + ly * ly > rcs * rcs) continue;
+                    stencil.push_back(ox);
+                    stencil.push_back(oy);
+                }
+            d.nstencil_b = static_cast<int>(stencil.size() / 2) - d.stencil_off_b;
+            d.cutb2 = rcs * rcs * (1.0 + 1e-5);
+            const double half = 0.5 * e.verlet_skin * (1.0 - 1e-4);
+            d.skin_half2 = half * half;
+            double vol = 1.0;
+            for (int k = 0; k < 3; k++) vol *= e.geom.dim[k] - e.geom.origin[k];
+            const double expect = (vol > 0 ? e.parts.size() / vol : 0.0) * 4.0 / 3.0 * M_PI * rcs * rcs * rcs;
+            const int cap = e.verlet_capacity > 0 ? e.verlet_capacity : static_cast<int>(2.0 * expect) + 64;
+            lcap = std::max(lcap, (cap + 31) / 32 * 32);
+        } else {
+            list_ok = false;
+        }
         d.cutoff = e.geom.cutoff;
         d.cutoff2 = e.geom.cutoff2;
         d.dt = e.dt;
@@ -339,8 +434,20 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     E->max_sys_blocks = (max_np + energy_tile() - 1) / energy_tile();
     int max_nt = 1;
     for (const SysDesc &d : sys) max_nt = std::max(max_nt, d.ntypes);
+    // Lists carry tables only (no DPD, flux or radius transforms), with every
+    // system's tables in LDS, and slots below 2^25.
+    E->list_mode = list_ok && !E->has_dpd && !E->has_flux && !E->has_radius && np < (1 << kListSlotBits) &&
+                   static_cast<size_t>(max_coef) * sizeof(real) <= 16384 && max_coef > 0;
+    E->lcap = E->list_mode ? lcap : 0;
     int rc = nonbond_setup<real>(E, max_nt, max_coef);
     if (rc != TFG_OK) return rc;
+    if (E->list_mode && E->coef_lds_bytes == 0) E->list_mode = false;
+    std::vector<int4> lblocks;
+    if (E->list_mode)
+        for (int s = 0; s < nsys; s++)
+            for (int f = 0; f < sys[s].np; f += 256)
+                lblocks.push_back(make_int4(s, sys[s].p_off + f, std::min(256, sys[s].np - f), 0));
+    E->nlblocks = static_cast<int>(lblocks.size());
 
     // Particles in id order (slot == global id) into buffer B, and the bonded
     // incidence lists (CSR by global particle id).
@@ -450,6 +557,16 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     ALLOC(E->inc, int, inc.size());
     ALLOC(E->d_bonds, int4, hb.size());
     ALLOC(E->d_angles, int4, ha.size());
+    ALLOC(E->d_lflags, int, 4);
+    ALLOC(E->d_sflags, int, 3 * nsys);
+    if (E->list_mode) {
+        ALLOC(E->d_list, int, (size_t)np * E->lcap);
+        ALLOC(E->d_lcount, int, np);
+        ALLOC(E->d_cid0, int, np);
+        ALLOC(E->d_lblock, int4, lblocks.size());
+        ALLOC(E->xb, v4, np);
+        ALLOC(E->x0, v4, np);
+    }
 #undef ALLOC
 
     auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
@@ -478,6 +595,14 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     if ((rc = H2D(E->inc, inc.data(), sizeof(int) * inc.size()))) return rc;
     if ((rc = H2D(E->d_bonds, hb.data(), sizeof(int4) * hb.size()))) return rc;
     if ((rc = H2D(E->d_angles, ha.data(), sizeof(int4) * ha.size()))) return rc;
+    if (E->list_mode && (rc = H2D(E->d_lblock, lblocks.data(), sizeof(int4) * lblocks.size()))) return rc;
+    {
+        const int flags[4] = {0, 0, 0, 0};
+        if ((rc = H2D(E->d_lflags, flags, sizeof(flags)))) return rc;
+        std::vector<int> sfl(3 * nsys, 0);
+        for (int s = 0; s < nsys; s++) sfl[s] = 1;   // the first step builds every system's lists
+        if ((rc = H2D(E->d_sflags, sfl.data(), sizeof(int) * sfl.size()))) return rc;
+    }
     // Zeroed state (outside any graph capture).
     TFG_HIP(hipMemset(E->d_step, 0, sizeof(long long)));
     TFG_HIP(hipMemset(E->d_err, 0, sizeof(int)));
@@ -489,7 +614,8 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     TFG_HIP(hipMemset(E->d_energy, 0, sizeof(double) * 5 * nsys));
     TFG_HIP(hipMemset(E->d_spec_tot, 0, sizeof(double) * kMaxSpecies * nsys));
     TFG_HIP(hipDeviceSynchronize());
-    if ((rc = run_count(E)) != TFG_OK) return rc;
+    // (list mode counts the cells when it rebuilds)
+    if (!E->list_mode && (rc = run_count(E)) != TFG_OK) return rc;
     TFG_HIP(hipStreamSynchronize(E->stream));
     return TFG_OK;
 }
@@ -631,6 +757,32 @@ template <typename real> static int step_t(DeviceEnsemble *E, int nsteps, bool u
         TFG_HIP(hipMemset(E->d_err, 0, sizeof(int)));
         return set_error(TFG_E_TOOFAST, "particle moving too fast (moved two cells or more in one step)");
     }
+    if (E->list_mode) {
+        int fl[4];
+        TFG_HIP(hipMemcpy(fl, E->d_lflags, sizeof(fl), hipMemcpyDeviceToHost));
+        if (fl[2]) {
+            fl[2] = 0;
+            TFG_HIP(hipMemcpy(E->d_lflags, fl, sizeof(fl), hipMemcpyHostToDevice));
+            std::vector<int> req(E->nsys, 1);   // every system rebuilds at the next step
+            TFG_HIP(hipMemcpy(E->d_sflags, req.data(), sizeof(int) * E->nsys, hipMemcpyHostToDevice));
+            return set_error(TFG_E_RANGE, "neighbour list capacity (" + std::to_string(E->lcap) +
+                                              " entries per particle) exceeded; raise it with tfg_engine_set_verlet");
+        }
+    }
+    return TFG_OK;
+}
+
+int ensemble_list_state(DeviceEnsemble *E, int sys, int *active, long long *rebuilds, int *capacity) {
+    if (active) *active = E->list_mode ? 1 : 0;
+    if (capacity) *capacity = E->lcap;
+    if (rebuilds) {
+        *rebuilds = 0;
+        if (E->list_mode && sys >= 0 && sys < E->nsys) {
+            int n = 0;
+            TFG_HIP(hipMemcpy(&n, E->d_sflags + 2 * E->nsys + sys, sizeof(int), hipMemcpyDeviceToHost));
+            *rebuilds = n;
+        }
+    }
     return TFG_OK;
 }
 
@@ -650,6 +802,8 @@ template <typename real> static int force_t(DeviceEnsemble *E) {
     KP<real> P = make_params<real>(E, 1);
     P.step_ahead = 1;   // the evaluation belongs to the next step (engine_step increments time first)
     int rc;
+    if (E->list_mode)
+        hipLaunchKernelGGL(k_list_rebin, dim3(1), dim3(256), 0, E->stream, E->d_lflags, E->d_sflags, E->nsys);
     TFG_HIP(hipEventRecord(E->ev[0], E->stream));
     if ((rc = run_bin<real>(E, P)) != TFG_OK) return rc;
     TFG_HIP(hipEventRecord(E->ev[1], E->stream));

@@ -54,7 +54,6 @@ constexpr int kWaveLdsBudget = 40960;    // staging bytes of a workgroup
 // b_inside << 22 (b is below the cutoff, so its end needs the guard too).
 // Staged particles carry their type as the byte offset of its descriptor
 // column (type * kPairBytes), the i particle its row (type * ntypes * kPairBytes).
-constexpr int kPairVecs = 4;
 constexpr int kMetaDpd = 1 << 16;
 constexpr int kMetaShifted = 1 << 20;
 constexpr int kMetaScaled = 1 << 21;
@@ -108,10 +107,12 @@ static inline int align16(int b) { return (b + 15) & ~15; }
 // Dynamic LDS layout: pair descriptors, table coefficients, then one
 // staging region per wave (positions, ids, velocities, radii, species).
 template <typename real>
-NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, bool radius, int max_chunk) {
+NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, bool radius, int max_chunk,
+                   bool build = false) {
     using v4 = typename V<real>::v4;
     const bool ids = dpd || flux;
-    const bool slots = std::is_same<real, float>::value;   // the cutoff guard's lookups
+    // the cutoff guard's lookups; the list build's entries
+    const bool slots = std::is_same<real, float>::value || build;
     NbLayout L{};
     L.waves = kNbWaves;
     L.off_coef = align16(max_nt * max_nt * kPairVecs * static_cast<int>(sizeof(v4)));
@@ -139,11 +140,59 @@ NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, b
     return L;
 }
 
-template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY>
+// Pair descriptor of type pair t of system sd (the layout above); off_coef
+// is the LDS byte offset of the system's tables when they are cached.
+template <typename real, bool CLDS>
+__device__ __forceinline__ void write_pair_descriptor(const KP<real> &P, const SysDesc &sd, int t, int off_coef,
+                                                      typename V<real>::v4 *s_pair) {
+    using v4 = typename V<real>::v4;
+    const int pi = P.potmap[sd.pot_off + t];
+    v4 q0, q1, q2, q3;
+    q0.x = q0.y = q0.z = q0.w = real(0);
+    q1 = q2 = q3 = q0;
+    q0.y = real(-1);   // absent: nothing in range
+    int meta = 0;
+    if (pi >= 0) {
+        const PotDesc<real> pd = P.pots[pi];
+        meta = (pd.kind == KIND_DPD ? kMetaDpd : 0) | ((pd.flags & POT_SHIFTED) ? kMetaShifted : 0) |
+               ((pd.flags & POT_SCALED) ? kMetaScaled : 0) |
+               (pd.bd < sd.cutoff * (1.0 - 4.0 * kGuard) || (pd.flags & (POT_SHIFTED | POT_SCALED)) ? kMetaBInside : 0);
+        q0.x = pd.a;
+        q0.y = pd.b;
+        q2.y = real(pd.bd);
+        q2.z = real(pd.bd - static_cast<double>(q2.y));
+        if (pd.kind == KIND_DPD) {
+            q0.z = pd.dalpha;
+            q0.w = pd.dgamma;
+            q1.x = pd.dsigma;
+            q1.y = real(1) / pd.b;
+        } else {
+            q0.z = pd.al0;
+            q0.w = pd.al1;
+            q1.x = pd.al2;
+            q1.y = real(pd.n);
+            q1.z = as_real_bits(CLDS ? off_coef + (pd.coff - sd.coef_begin) * static_cast<int>(sizeof(real)) : pd.coff,
+                                real(0));
+        }
+        q2.x = pd.r0p1;
+    }
+    q1.w = as_real_bits(meta, real(0));
+    s_pair[kPairVecs * t] = q0;
+    s_pair[kPairVecs * t + 1] = q1;
+    s_pair[kPairVecs * t + 2] = q2;
+    s_pair[kPairVecs * t + 3] = q3;
+}
+
+// BUILD: the neighbour-list build.  The same staging and distance test over
+// the stencil of cutoff + skin; the set bits are written to the lists
+// (partner slot and build-cell offset, in candidate order) instead of being
+// evaluated.  It runs only in the steps that rebuild.
+template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY, bool BUILD = false>
 __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
     using v4 = typename V<real>::v4;
     constexpr bool kIds = DPD || FLUX;   // pair terms that need the partner's id
-    constexpr bool kGuarded = std::is_same<real, float>::value;
+    constexpr bool kGuarded = std::is_same<real, float>::value && !BUILD;
+    if (BUILD && P.lflags[0] == 0) return;
     constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ FluxDesc<real> s_flux[FLUX ? kMaxFluxEntries : 1];
@@ -177,47 +226,15 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
         const int sysid = P.cell_sys[seg];
         const SysDesc &sd = P.sys[sysid];
         const int seg_end = min(c_end, sd.cell_off + sd.ncells);
+        if (BUILD && P.sflags[P.nsys + sysid] == 0) {   // this system keeps its lists
+            seg = seg_end;
+            continue;
+        }
         const int nt = sd.ntypes;
         // Pair descriptors (and tables) of this system.
         __syncthreads();
         for (int t = threadIdx.x; t < nt * nt; t += kNbThreads) {
-            const int pi = P.potmap[sd.pot_off + t];
-            v4 q0, q1, q2, q3;
-            q0.x = q0.y = q0.z = q0.w = real(0);
-            q1 = q2 = q3 = q0;
-            q0.y = real(-1);   // absent: nothing in range
-            int meta = 0;
-            if (pi >= 0) {
-                const PotDesc<real> pd = P.pots[pi];
-                meta = (pd.kind == KIND_DPD ? kMetaDpd : 0) | ((pd.flags & POT_SHIFTED) ? kMetaShifted : 0) |
-                       ((pd.flags & POT_SCALED) ? kMetaScaled : 0) |
-                       (pd.bd < sd.cutoff * (1.0 - 4.0 * kGuard) || (pd.flags & (POT_SHIFTED | POT_SCALED)) ? kMetaBInside
-                                                                                                          : 0);
-                q0.x = pd.a;
-                q0.y = pd.b;
-                q2.y = real(pd.bd);
-                q2.z = real(pd.bd - static_cast<double>(q2.y));
-                if (pd.kind == KIND_DPD) {
-                    q0.z = pd.dalpha;
-                    q0.w = pd.dgamma;
-                    q1.x = pd.dsigma;
-                    q1.y = real(1) / pd.b;
-                } else {
-                    q0.z = pd.al0;
-                    q0.w = pd.al1;
-                    q1.x = pd.al2;
-                    q1.y = real(pd.n);
-                    q1.z = as_real_bits(CLDS ? P.nb.off_coef + (pd.coff - sd.coef_begin) * static_cast<int>(sizeof(real))
-                                             : pd.coff,
-                                        real(0));
-                }
-                q2.x = pd.r0p1;
-            }
-            q1.w = as_real_bits(meta, real(0));
-            s_pair[kPairVecs * t] = q0;
-            s_pair[kPairVecs * t + 1] = q1;
-            s_pair[kPairVecs * t + 2] = q2;
-            s_pair[kPairVecs * t + 3] = q3;
+            write_pair_descriptor<real, CLDS>(P, sd, t, P.nb.off_coef, s_pair);
             if (FLUX) s_fmap[t] = make_int2(P.fluxmap[2 * (sd.flux_off + t)], P.fluxmap[2 * (sd.flux_off + t) + 1]);
         }
         if (FLUX)
@@ -228,7 +245,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
 
         // FP32: the test accepts up to rc^2 (1 + kGuard); pairs above rc^2 (1 - kGuard)
         // are decided again in FP64.  FP64: rc^2 itself.
-        const real cut2 = kGuarded ? real(sd.cutoff2 * (1.0 + kGuard)) : real(sd.cutoff2);
+        const real cut2 = BUILD ? real(sd.cutb2) : (kGuarded ? real(sd.cutoff2 * (1.0 + kGuard)) : real(sd.cutoff2));
         const real cut2_lo = kGuarded ? real(sd.cutoff2 * (1.0 - kGuard)) : real(sd.cutoff2);
         const bool binside = kGuarded && sd.binside != 0;   // a table of this system ends below the cutoff
         const float ihf[3] = {float(1.0 / sd.h[0]), float(1.0 / sd.h[1]), float(1.0 / sd.h[2])};
@@ -272,6 +289,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                 if (FLUX)
                     for (int k = 0; k < S; k++) qacc[k] = 0;
                 int self_k = -1;   // staged index of the i particle itself in the current chunk
+                int lbase = 0;     // BUILD: list entries of this i particle so far
 
                 // One candidate k of the staged list; ok = false makes it a no-op
                 // (k is then 0, a staged entry, so every read stays in range).
@@ -431,6 +449,38 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                         const int st = self_k - kb;
                         if (st >= 0 && st < 64) m0 &= ~(1ull << st);
                         if (st >= 64 && st < 128) m1 &= ~(1ull << (st - 64));
+                        if constexpr (BUILD) {
+                            // Entries of the slices of one i particle follow each
+                            // other in slice order: offsets from a wave-wide scan
+                            // of the counts, less the scan before its first slice.
+                            const int c = __popcll(m0) + __popcll(m1);
+                            int inc = c;
+#pragma unroll
+                            for (int o = 1; o < kWave; o <<= 1) {
+                                const int y = __shfl_up(inc, o);
+                                if (lane >= o) inc += y;
+                            }
+                            const int seg0 = il * slices;
+                            const int before = __shfl(inc, seg0 > 0 ? seg0 - 1 : 0);
+                            const int last = __shfl(inc, min(seg0 + slices - 1, kWave - 1));
+                            const int b0 = seg0 > 0 ? before : 0;
+                            int pos = lbase + inc - c - b0;
+                            const int *codes = s_slot + kb;
+                            while (m0) {
+                                const int t = __builtin_ctzll(m0);
+                                m0 &= m0 - 1ull;
+                                if (pos < P.lcap) P.list[(size_t)pos * P.np + si] = codes[t];
+                                pos++;
+                            }
+                            while (m1) {
+                                const int t = __builtin_ctzll(m1);
+                                m1 &= m1 - 1ull;
+                                if (pos < P.lcap) P.list[(size_t)pos * P.np + si] = codes[64 + t];
+                                pos++;
+                            }
+                            lbase += last - b0;
+                            continue;
+                        }
                         // Evaluate the set bits, two per iteration.  m0 is refilled
                         // from m1 when it runs empty; an empty pick gives ok = false.
                         int base = kb;
@@ -503,10 +553,12 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                 };
 
                 int nstaged = 0;
-                const int sz = sd.span_z;
-                for (int col = 0; col < sd.nstencil; col++) {
-                    const int ox = P.stencil[2 * (sd.stencil_off + col)];
-                    const int oy = P.stencil[2 * (sd.stencil_off + col) + 1];
+                const int sz = BUILD ? sd.span_z_b : sd.span_z;
+                const int nst = BUILD ? sd.nstencil_b : sd.nstencil;
+                const int st_off = BUILD ? sd.stencil_off_b : sd.stencil_off;
+                for (int col = 0; col < nst; col++) {
+                    const int ox = P.stencil[2 * (st_off + col)];
+                    const int oy = P.stencil[2 * (st_off + col) + 1];
                     int n0 = loc[0] + ox;
                     int n1 = loc[1] + oy;
                     // beyond a non-periodic face there is no neighbour
@@ -566,6 +618,10 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                                 s_pos[nstaged + t] = p;
                                 if (kIds) s_id[nstaged + t] = mj.y;
                                 if (kGuarded) s_slot[nstaged + t] = sj;
+                                if (BUILD)
+                                    s_slot[nstaged + t] = static_cast<int>(static_cast<unsigned>(sj) |
+                                                                           (static_cast<unsigned>(list_code(ox, oy, oz))
+                                                                            << kListSlotBits));
                                 if (DPD) s_vel[nstaged + t] = P.A.vel[sj];
                                 if (FLUX)
                                     for (int k = 0; k < S; k++)
@@ -588,6 +644,13 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     wave_sync();
                     process(nstaged);
                     wave_sync();
+                }
+                if constexpr (BUILD) {
+                    if (active && sl == 0) {
+                        P.lcount[si] = lbase;
+                        if (lbase > P.lcap) atomicOr(&P.lflags[2], 1);
+                    }
+                    continue;
                 }
                 // Combine the slices of each i particle in slice order.
                 if (slices > 1) {
@@ -651,6 +714,11 @@ __global__ __launch_bounds__(kNbThreads) __attribute__((amdgpu_waves_per_eu(5, 8
     nonbond_body<float, false, false, RADIUS, CLDS, ENERGY>(P);
 }
 
+template <typename real>
+__global__ __launch_bounds__(kNbThreads) void k_list_build(KP<real> P) {
+    nonbond_body<real, false, false, false, false, false, true>(P);
+}
+
 template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY>
 static const void *nonbond_kernel() {
     if constexpr (std::is_same<real, float>::value && !DPD && !FLUX)
@@ -697,6 +765,14 @@ template <typename real> int nonbond_setup(DeviceEnsemble *E, int max_nt, int ma
     TFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNbThreads, E->nb.total));
     if (per_cu < 1) return set_error(TFG_E_DEVICE, "pair kernel does not fit a compute unit");
     E->nonbond_resident = per_cu * E->cus;
+    if (E->list_mode) {
+        E->nbb = nb_layout<real>(max_nt, 0, E->S, false, false, false, max_chunk, true);
+        // list kernel: descriptors, tables, 125 build-cell offsets
+        E->nbl = NbLayout{};
+        E->nbl.off_coef = align16(max_nt * max_nt * kPairVecs * static_cast<int>(sizeof(typename V<real>::v4)));
+        E->nbl.off_waves = E->nbl.off_coef + align16(max_coef * static_cast<int>(sizeof(real)));
+        E->nbl.total = E->nbl.off_waves + 125 * static_cast<int>(sizeof(typename V<real>::v4));
+    }
     return TFG_OK;
 }
 
@@ -711,6 +787,177 @@ template <typename real> int run_nonbond(DeviceEnsemble *E, const KP<real> &P) {
     return launch_error("nonbond");
 }
 
+// The list build runs in every step of list mode and returns at once unless
+// the step rebuilds (the decision is on the device, so the step graph is
+// fixed).
+template <typename real> int run_list_build(DeviceEnsemble *E, const KP<real> &P) {
+    KP<real> Q = P;
+    Q.nb = E->nbb;
+    void *args[] = {&Q};
+    const hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(&k_list_build<real>), dim3(E->nonbond_grid()),
+                                         dim3(kNbThreads), args, E->nbb.total, E->stream);
+    if (e != hipSuccess) return set_error(TFG_E_DEVICE, std::string("list build: ") + hipGetErrorString(e));
+    return launch_error("list build");
+}
+
+
+// FP64 squared distance of particles i and j from their stored positions and
+// current cells (the list kernel's cutoff guard): the cell offset is taken
+// to the nearest image on periodic axes (partners are at most two cells
+// apart and periodic list axes have at least five cells).
+__device__ inline double list_guard_r2(const KP<float> &P, const SysDesc &sd, int i, int j) {
+    const float4 a = P.A.pos[i], b = P.A.pos[j];
+    int li[3], lj[3];
+    cell_loc(P.A.meta[i].z - sd.cell_off, sd, li);
+    cell_loc(P.A.meta[j].z - sd.cell_off, sd, lj);
+    const float av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z};
+    double r2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        int d = lj[k] - li[k];
+        if (sd.periodic & (1 << k)) {
+            if (2 * d > sd.cdim[k]) d -= sd.cdim[k];
+            else if (2 * d < -sd.cdim[k]) d += sd.cdim[k];
+        }
+        const double e = (static_cast<double>(av[k]) - static_cast<double>(bv[k])) - static_cast<double>(d) * sd.h[k];
+        r2 = __builtin_fma(e, e, r2);
+    }
+    return r2;
+}
+__device__ inline double list_guard_r2(const KP<double> &, const SysDesc &, int, int) { return 0.0; }
+
+// Pair forces from the neighbour lists: one lane per particle, entries in
+// list order (the build's candidate order), so each particle's sum has a
+// fixed order.  Tables only (no DPD, flux or radius transforms: those
+// systems use the cell kernel).  A workgroup covers up to kNbThreads
+// particles of one system and caches its pair descriptors, tables and the
+// 125 build-cell offsets in LDS.  The pair test, the FP32 cutoff guard
+// (band pairs decided in FP64 from the stored positions), the number density
+// and the table evaluation are those of the cell kernel.
+template <typename real, bool ENERGY>
+__global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
+    using v4 = typename V<real>::v4;
+    constexpr bool kGuarded = std::is_same<real, float>::value;
+    constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    const int4 lb = P.lblock[blockIdx.x];
+    const SysDesc &sd = P.sys[lb.x];
+    const int nt = sd.ntypes;
+    v4 *s_pair = reinterpret_cast<v4 *>(s_dyn);
+    real *s_coef = reinterpret_cast<real *>(s_dyn + P.nb.off_coef);
+    v4 *s_sh = reinterpret_cast<v4 *>(s_dyn + P.nb.off_waves);
+    for (int t = threadIdx.x; t < nt * nt; t += kNbThreads) write_pair_descriptor<real, true>(P, sd, t, P.nb.off_coef, s_pair);
+    for (int t = threadIdx.x; t < sd.coef_count; t += kNbThreads) s_coef[t] = P.coef[sd.coef_begin + t];
+    for (int c = threadIdx.x; c < 125; c += kNbThreads) {
+        v4 sh;
+        sh.x = real((c / 25 - 2) * sd.h[0]);
+        sh.y = real((c / 5 % 5 - 2) * sd.h[1]);
+        sh.z = real((c % 5 - 2) * sd.h[2]);
+        sh.w = real(0);
+        s_sh[c] = sh;
+    }
+    __syncthreads();
+
+    const bool valid = static_cast<int>(threadIdx.x) < lb.z;
+    const int i = lb.y + (valid ? static_cast<int>(threadIdx.x) : 0);
+    const real cut2 = kGuarded ? real(sd.cutoff2 * (1.0 + kGuard)) : real(sd.cutoff2);
+    const real cut2_lo = kGuarded ? real(sd.cutoff2 * (1.0 - kGuard)) : real(sd.cutoff2);
+    const bool binside = kGuarded && sd.binside != 0;
+    const real w0 = real(sd.w_self);
+    const real w2 = real(-1.5 * sd.w_self / sd.cutoff2);
+    const real w3 = real(0.75 * sd.w_self / (sd.cutoff2 * sd.cutoff));
+    const real eps2 = eps_of<real>() * eps_of<real>();
+
+    const v4 pi = P.xb[i];
+    const int ti = P.A.meta[i].x & 0xffff;
+    const int ti_row = ti * nt * kPairBytes;
+    const int n = valid ? min(P.lcount[i], P.lcap) : 0;
+    real fx = 0, fy = 0, fz = 0, nd = 0, ep = 0, vir = 0;
+    for (int k = 0; __any(k < n); k++) {
+        const bool live = k < n;
+        const unsigned e = live ? static_cast<unsigned>(P.list[(size_t)k * P.np + i]) : static_cast<unsigned>(i);
+        const int j = static_cast<int>(e & kListSlotMask);
+        const v4 sh = s_sh[live ? (e >> kListSlotBits) : 62];
+        v4 pj = P.xb[j];
+        pj.x = pj.x + sh.x;
+        pj.y = pj.y + sh.y;
+        pj.z = pj.z + sh.z;
+        const real dx = pi.x - pj.x;
+        const real dy = pi.y - pj.y;
+        const real dz = pi.z - pj.z;
+        const real r2 = dist2(pi, pj);
+        bool ok = live && r2 <= cut2;
+        const real ir = rsq(fmax(r2, eps2));
+        const real r_raw = r2 * ir;
+        const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + ti_row + real_bits_as_int(pj.w));
+        const v4 q0 = qd[0];
+        const v4 q1 = qd[1];
+        bool in = ok && r_raw <= q0.y;
+        if (kGuarded) {
+            const int meta = real_bits_as_int(q1.w);
+            bool band = ok && r2 > cut2_lo;
+            if (binside) band = band || (ok && (meta & kMetaBInside) && fabs(r_raw - q0.y) <= real(2.0 * kGuard) * q0.y);
+            if (__builtin_expect(band, false)) {
+                const double r2d = list_guard_r2(P, sd, i, j);
+                const v4 q2 = qd[2];
+                const double bd = static_cast<double>(q2.y) + static_cast<double>(q2.z);
+                ok = r2d <= sd.cutoff2;
+                in = ok && r2d <= bd * bd;
+            }
+        }
+        nd += ok ? w0 + r2 * (w2 + r_raw * w3) : real(0);
+        const real r = fmax(r_raw, q0.x);
+        const int ind = static_cast<int>(clamp_real(q0.z + r * (q0.w + r * q1.x), real(1), q1.y));
+        const int tb = real_bits_as_int(q1.z);
+        const v4 *c4 = reinterpret_cast<const v4 *>(s_dyn + tb + ind * 8 * static_cast<int>(sizeof(real)));
+        const v4 c0 = c4[0];
+        const v4 c1 = c4[1];
+        const real x = (r - c0.x) * c0.y;
+        real ee = c0.z * x + c0.w;
+        real eff = c0.z;
+        eff = eff * x + ee; ee = ee * x + c1.x;
+        eff = eff * x + ee; ee = ee * x + c1.y;
+        eff = eff * x + ee; ee = ee * x + c1.z;
+        eff = eff * x + ee; ee = ee * x + c1.w;
+        const real f = in ? eff * c0.y * ir : real(0);
+        fx -= f * dx;
+        fy -= f * dy;
+        fz -= f * dz;
+        if (ENERGY) {
+            ep += in ? ee : real(0);
+            vir -= f * r2;
+        }
+    }
+    if (valid) {
+        v4 fo;
+        fo.x = fx;
+        fo.y = fy;
+        fo.z = fz;
+        fo.w = nd + real(sd.w_self);
+        P.force[i] = fo;
+        if (ENERGY) {
+            P.e_nb[i] = 0.5 * static_cast<double>(ep);
+            P.e_vir[i] = 0.5 * static_cast<double>(vir);
+        }
+    }
+}
+
+template <typename real> int run_list_force(DeviceEnsemble *E, const KP<real> &P) {
+    KP<real> Q = P;
+    Q.nb = E->nbl;
+    void *args[] = {&Q};
+    const void *fn = P.energy ? reinterpret_cast<const void *>(&k_list_force<real, true>)
+                              : reinterpret_cast<const void *>(&k_list_force<real, false>);
+    const hipError_t e = hipLaunchKernel(fn, dim3(E->nlblocks), dim3(kNbThreads), args, E->nbl.total, E->stream);
+    if (e != hipSuccess) return set_error(TFG_E_DEVICE, std::string("list pairs: ") + hipGetErrorString(e));
+    return launch_error("list pairs");
+}
+
+template int run_list_force<float>(DeviceEnsemble *, const KP<float> &);
+template int run_list_force<double>(DeviceEnsemble *, const KP<double> &);
+
+template int run_list_build<float>(DeviceEnsemble *, const KP<float> &);
+template int run_list_build<double>(DeviceEnsemble *, const KP<double> &);
 template int run_nonbond<float>(DeviceEnsemble *, const KP<float> &);
 template int run_nonbond<double>(DeviceEnsemble *, const KP<double> &);
 template int nonbond_setup<float>(DeviceEnsemble *, int, int);

@@ -161,6 +161,31 @@ int tfg_engine_set_wall_potential(tfg_engine *h, int face, int type, const tfg_p
     }));
 }
 
+int tfg_engine_set_verlet(tfg_engine *h, double skin, int capacity) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    if (!(skin >= 0) || !std::isfinite(skin)) return set_error(TFG_E_INVALIDARG, "skin must be finite and >= 0");
+    if (capacity < 0) return set_error(TFG_E_INVALIDARG, "capacity must be >= 0");
+    TFG_GUARD(return with_refresh(h, [&] {
+        h->e.verlet_skin = skin;
+        h->e.verlet_capacity = capacity;
+        return TFG_OK;
+    }));
+}
+
+int tfg_engine_verlet_state(const tfg_engine *h, int *active, int64_t *rebuilds, int *capacity) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    if (active) *active = 0;
+    if (rebuilds) *rebuilds = 0;
+    if (capacity) *capacity = 0;
+    if (!h->e.device) return TFG_OK;
+    long long r = 0;
+    int rc;
+    TFG_GUARD(rc = ensemble_list_state(h->e.device, h->e.device_index, active, &r, capacity));
+    if (rc != TFG_OK) return rc;
+    if (rebuilds) *rebuilds = r;
+    return TFG_OK;
+}
+
 int tfg_engine_set_nspecies(tfg_engine *h, int nspecies) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
     if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");

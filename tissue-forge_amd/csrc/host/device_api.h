@@ -43,5 +43,7 @@ int ensemble_precision(DeviceEnsemble *ens);
 int ensemble_size(DeviceEnsemble *ens);
 // Per-step phase timing with HIP events on the module's stream (eager steps).
 int ensemble_set_profile(DeviceEnsemble *ens, bool on);
+// Neighbour lists: in use, rebuilds of system sys so far, entries per particle.
+int ensemble_list_state(DeviceEnsemble *ens, int sys, int *active, long long *rebuilds, int *capacity);
 
 }  // namespace tfg

@@ -129,6 +129,10 @@ class Engine {
     double max_dist_fraction = 0.05;   // tfEngine.cpp:1865
     int integrator = INTEG_FORWARD_EULER;
     BoundaryFace bc_set[6];        // faces as set (every face periodic by default)
+    // Neighbour lists with a skin (0: off); capacity in entries per particle
+    // (0: from the density).  The pair set of every step is unchanged.
+    double verlet_skin = 0.0;
+    int verlet_capacity = 0;
     std::vector<int> wallpot;      // 6 * max_types: pot id of (face, type) or -1
     int64_t time = 0;                  // step counter (engine->time)
     int dpd_noise = DPD_NOISE_UNIT;

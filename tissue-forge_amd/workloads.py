@@ -207,6 +207,8 @@ def build_engine(scene, potentials=None, max_distance_fraction=None):
         eng.set_wall_potential(face, t, pots[pi])
     if scene.get("dpd_noise"):
         eng.set_dpd_noise(*scene["dpd_noise"])
+    if scene.get("verlet_skin"):
+        eng.set_verlet(scene["verlet_skin"], scene.get("verlet_capacity", 0))
     eng.add_particles(scene["ptype"], scene["x"], scene["v"], scene["species"])
     if scene.get("bonds") is not None:
         eng.add_bonds(pots[scene["bonds"]["pot"]], scene["bonds"]["ij"])
