@@ -3,15 +3,19 @@
 Workload: the 2,000,000-particle Morse+LJ system (4 types, density 1,
 cutoff 2.5, tabulated potentials), FP32-mixed path.  One step = binning,
 nonbonded pass, bonded pass, forward Euler with migration, all on the GPU
-with the particles resident in HBM.
+with the particles resident in HBM.  By default the pair pass uses neighbour
+lists with a 0.3 skin (rebuilt on the device when a particle has moved more
+than half the skin; the pair set of every step is unchanged); the cell pass
+is timed beside it.
 
-Prints one JSON line with particle-steps/s, the FP64 GPU path beside it,
-the force error of both paths against the FP64 CPU oracle, the roofline of
-the nonbonded kernel (HIP events on the module's stream around every phase of
-every timed step), the CPU baseline (the oracle restatement of the reference
-CPU engine, timed on a bounded number of steps on this host's cores) and the
-512-replica ensemble, batched in one launch sequence against the same
-replicas stepped one after another.
+Prints one JSON line with particle-steps/s (a multi-step call: the captured
+step graph replayed), the phase split of the same steps run eagerly with HIP
+events on the module's stream, the cell pass and the FP64 GPU path beside
+it, the force error of both precisions against the FP64 CPU oracle, the
+roofline of the dominant pair kernel, the CPU baseline (the oracle
+restatement of the reference CPU engine, timed on a bounded number of steps
+on this host's cores) and the 512-replica ensemble, batched in one launch
+sequence against the same replicas stepped one after another.
 
 --calibrate-copy runs only a plain device copy of known size (the reference
 for scaling the FETCH_SIZE / WRITE_SIZE counters of a profiler run).
@@ -35,13 +39,18 @@ HBM_PEAK_GBS = 8000.0
 # particle's 16-byte position record is read by the 27 cells of its stencil,
 # plus its own force write (27*16 + 16).
 NONBOND_BYTES_PER_PARTICLE = 448.0
+# Compulsory bytes per particle of the list pair kernel besides its list
+# entries (4 B each): its own position and record, its entry count, and the
+# force written (16 + 16 + 4 + 16).
+LIST_FIXED_BYTES = 52.0
+PHASES = ("bin_avg", "build_avg", "nonbond_avg", "advance_avg")
 
 
-def traffic_bytes():
-    """HBM bytes per launch of the nonbonded kernel on this workload, from the
+def traffic_bytes(kernel):
+    """HBM bytes per launch of a pair kernel on this workload, from the
     committed profiler summary (FETCH_SIZE + WRITE_SIZE, each scaled by the
     same counter's reading for a plain copy of known size); None without it."""
-    path = os.path.join(ROOT, "profiles", "k_nonbond_morse_lj_2M_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "%s_morse_lj_2M_traffic.txt" % kernel)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -72,29 +81,39 @@ def force_errors(fg, fo):
     return dict(err_rms=float(d.max() / scale), rel_p99=float(np.percentile(rel, 99)), rel_max=float(rel.max()))
 
 
-def time_gpu(tf, W, scene, pots, precision, steps, warmup):
+def time_gpu(tf, W, scene, pots, precision, steps, warmup, skin=0.0):
     """Timed region: `steps` whole steps with HIP events around each phase
-    (bin, nonbond, advance) on the module's stream; then the same number of
-    steps replayed from the captured graph, without events."""
+    (bin, list build, pair pass, advance) on the module's stream; then the
+    same number of steps replayed from the captured graph, without events.
+    skin > 0 steps with neighbour lists."""
     eng, _ = W.build_engine(scene, potentials=pots)
+    eng.set_verlet(skin)
     eng.to_device(precision)
     shape = eng.autotune(reps=2)   # result-invariant launch shapes (tfg_autotune)
-    eng.force()
-    forces = eng.particles()["f"]
     xq = eng.positions()   # the input positions as the path holds them
+    eng.step(1)
+    forces = eng.particles()["f"]   # the first step's pair pass, evaluated at xq
     eng.step(max(warmup, 1))
+    r0 = eng.verlet_state()[1]
     eng.set_profiling(True)
     t0 = time.perf_counter()
     eng.step(steps)
     wall = time.perf_counter() - t0
     kt = eng.kernel_times()
+    active, r1, cap = eng.verlet_state()
+    lists = None
+    if active:
+        mean, mx = eng.verlet_entries()
+        lists = {"skin": skin, "rebuilds_in_timed_steps": r1 - r0, "capacity": cap, "mean_entries": mean,
+                 "max_entries": mx}
     eng.set_profiling(False)
     t0 = time.perf_counter()
     eng.step(steps)
     wall_graph = time.perf_counter() - t0
     eng.from_device()
     eng.close()
-    return dict(wall_s=wall, wall_graph_s=wall_graph, kernels=kt, forces=forces, positions=xq, shape=shape)
+    return dict(wall_s=wall, wall_graph_s=wall_graph, kernels=kt, forces=forces, positions=xq, shape=shape,
+                lists=lists)
 
 
 def bench_ensemble(tf, W, nrep, steps, warmup):
@@ -155,6 +174,8 @@ def main():
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--ens-steps", type=int, default=50)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--skin", type=float, default=0.3, help="neighbour-list skin (0: cell pass only)")
+    ap.add_argument("--no-cell", action="store_true", help="skip the cell-pass comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X, multi-GPU runs are not part "
@@ -175,18 +196,38 @@ def main():
     pots = [W.make_potential(p) for p in scene["pots"]]
     log("workload morse_lj n=%d L=%.3f cells=%s" % (n, scene["dim"][0], scene["cells"]))
 
-    mixed = time_gpu(tf, W, scene, pots, tf.PREC_MIXED, args.steps, args.warmup)
-    log("mixed: wall %.3f s for %d steps (graph replay %.3f s), kernels %s" %
-        (mixed["wall_s"], args.steps, mixed["wall_graph_s"], mixed["kernels"]))
+    mixed = time_gpu(tf, W, scene, pots, tf.PREC_MIXED, args.steps, args.warmup, args.skin)
+    log("mixed: wall %.3f s for %d steps (graph replay %.3f s), kernels %s, lists %s" %
+        (mixed["wall_s"], args.steps, mixed["wall_graph_s"], mixed["kernels"], mixed["lists"]))
+    cell = None
+    if args.skin > 0 and not args.no_cell:
+        cell = time_gpu(tf, W, scene, pots, tf.PREC_MIXED, args.steps, args.warmup, 0.0)
+        log("mixed cell pass: wall %.3f s (graph replay %.3f s), kernels %s" %
+            (cell["wall_s"], cell["wall_graph_s"], cell["kernels"]))
     fp64 = None
     if not args.no_fp64:
-        fp64 = time_gpu(tf, W, scene, pots, tf.PREC_FP64, args.steps, args.warmup)
-        log("fp64: wall %.3f s, kernels %s" % (fp64["wall_s"], fp64["kernels"]))
+        fp64 = time_gpu(tf, W, scene, pots, tf.PREC_FP64, args.steps, args.warmup, args.skin)
+        log("fp64: wall %.3f s (graph replay %.3f s), kernels %s" % (fp64["wall_s"], fp64["wall_graph_s"],
+                                                                    fp64["kernels"]))
 
-    value = n * args.steps / mixed["wall_s"]
-    ms_per_step = mixed["wall_s"] / args.steps * 1e3
-    nb_ms = mixed["kernels"]["nonbond_avg"]
-    achieved = NONBOND_BYTES_PER_PARTICLE * n / (nb_ms * 1e-3) / 1e9
+    # The timed region is a multi-step call, which replays the captured step
+    # graph (the library's path for several steps); the phase split comes
+    # from the same number of steps run eagerly with HIP events.
+    value = n * args.steps / mixed["wall_graph_s"]
+    ms_per_step = mixed["wall_graph_s"] / args.steps * 1e3
+    lists = mixed["lists"]
+    if lists is not None:
+        # dominant kernel: the list pair pass; its compulsory HBM bytes per
+        # particle are its own position, record, count and force plus the
+        # list entries (partner positions come from L2 / MALL)
+        nb_ms = mixed["kernels"]["nonbond_avg"]
+        bytes_pp = LIST_FIXED_BYTES + 4.0 * lists["mean_entries"]
+        kernel = "k_list_force"
+    else:
+        nb_ms = mixed["kernels"]["nonbond_avg"]
+        bytes_pp = NONBOND_BYTES_PER_PARTICLE
+        kernel = "k_nonbond"
+    achieved = bytes_pp * n / (nb_ms * 1e-3) / 1e9
 
     out = {
         "metric": "particle-steps/s at 1 MI355X (FP32-mixed vs FP64, force err <=1e-5); HBM roofline %",
@@ -204,18 +245,28 @@ def main():
         "config": {"workload": "morse_lj_2M: %d particles, 4 types, density 1, cutoff 2.5, Morse like / LJ unlike, "
                                "cells %s" % (n, scene["cells"]),
                    "parallelism": "single GPU"},
-        "phases_ms": {k: mixed["kernels"][k] for k in ("bin_avg", "nonbond_avg", "advance_avg")},
-        "graph_replay_value": n * args.steps / mixed["wall_graph_s"],
+        "phases_ms": {k: mixed["kernels"][k] for k in PHASES},
+        "evented_value": n * args.steps / mixed["wall_s"],
+        "neighbour_lists": lists,
         "autotuned_workgroups": {"pair": mixed["shape"][0], "stream": mixed["shape"][1]},
-        "roofline": {"bound": "hbm", "kernel": "k_nonbond", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(),
-                     "bytes_per_particle": NONBOND_BYTES_PER_PARTICLE, "avg_launch_ms": nb_ms},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(kernel),
+                     "bytes_per_particle": bytes_pp, "avg_launch_ms": nb_ms},
     }
+    if cell is not None:
+        cnb = cell["kernels"]["nonbond_avg"]
+        ca = NONBOND_BYTES_PER_PARTICLE * n / (cnb * 1e-3) / 1e9
+        out["cell_path"] = {"value": n * args.steps / cell["wall_graph_s"],
+                            "ms_per_step": cell["wall_graph_s"] / args.steps * 1e3,
+                            "phases_ms": {k: cell["kernels"][k] for k in PHASES},
+                            "roofline": {"kernel": "k_nonbond", "achieved": ca, "frac": ca / HBM_PEAK_GBS,
+                                         "traffic": traffic_bytes("k_nonbond"),
+                                         "bytes_per_particle": NONBOND_BYTES_PER_PARTICLE, "avg_launch_ms": cnb}}
     if fp64 is not None:
-        v64 = n * args.steps / fp64["wall_s"]
-        out["fp64_path"] = {"value": v64, "ms_per_step": fp64["wall_s"] / args.steps * 1e3,
-                            "phases_ms": {k: fp64["kernels"][k] for k in ("bin_avg", "nonbond_avg", "advance_avg")},
-                            "mixed_over_fp64": value / v64}
+        v64 = n * args.steps / fp64["wall_graph_s"]
+        out["fp64_path"] = {"value": v64, "ms_per_step": fp64["wall_graph_s"] / args.steps * 1e3,
+                            "phases_ms": {k: fp64["kernels"][k] for k in PHASES},
+                            "neighbour_lists": fp64["lists"], "mixed_over_fp64": value / v64}
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -117,6 +117,10 @@ int tfg_engine_set_verlet(tfg_engine *e, double skin, int capacity);
 /* Whether the device copy steps with lists, the rebuilds so far and the
  * capacity; zeros when the engine is not on the device. */
 int tfg_engine_verlet_state(const tfg_engine *e, int *active, int64_t *rebuilds, int *capacity);
+/* Mean and largest number of list entries per particle at the last build
+ * (zeros without lists).  No counterpart in the reference: diagnostics of
+ * the list sizes for choosing the skin and the capacity. */
+int tfg_engine_verlet_entries(const tfg_engine *e, double *mean, int *maxn);
 /* Species count per particle (the state vector of source/state/, flat here). */
 int tfg_engine_set_nspecies(tfg_engine *e, int nspecies);
 int tfg_engine_set_dpd_noise(tfg_engine *e, int mode, uint64_t seed);
@@ -225,11 +229,11 @@ int tfg_read_particles(tfg_engine *e, int32_t *cid, double *x, double *v, double
 int tfg_read_positions(tfg_engine *e, double *xg);
 /* FP64 per-species totals (flux conservation); returns the count. */
 int tfg_read_species_totals(tfg_engine *e, double *out, int cap);
-/* Device time of the last tfg_engine_step call in ms, and the per-kernel split
- * (bin, nonbond, advance) of its last step; returns the number of entries. */
+/* Device time of the last tfg_engine_step call in ms. */
 double tfg_last_step_ms(const tfg_engine *e);
-/* Per-phase device time: [bin, nonbond, advance] of the last step, then the
- * averages over all steps of the last call when profiling is on. */
+/* Per-phase device time: [bin, list build, pair pass, advance] of the last
+ * step, then the averages over all steps of the last call when profiling is
+ * on (the list build is 0 without neighbour lists); returns the count. */
 int tfg_kernel_times(const tfg_engine *e, double *ms, int cap);
 /* Records HIP events around every phase of every step (the role of the
  * reference's engine timers, source/mdcore/include/tfEngine.h:95-145). */

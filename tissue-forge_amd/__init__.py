@@ -74,6 +74,7 @@ _SIGNATURES = {
                                                ctypes.c_double]),
     "tfg_engine_set_wall_potential": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
     "tfg_engine_set_verlet": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int]),
+    "tfg_engine_verlet_entries": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int)]),
     "tfg_engine_verlet_state": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
                                                ctypes.POINTER(ctypes.c_int)]),
     "tfg_engine_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64]),
@@ -327,6 +328,13 @@ class Engine:
         _check(lib().tfg_engine_verlet_state(self._h, ctypes.byref(a), ctypes.byref(r), ctypes.byref(c)))
         return bool(a.value), r.value, c.value
 
+    def verlet_entries(self):
+        """(mean, largest) list entries per particle at the last build."""
+        m = ctypes.c_double()
+        x = ctypes.c_int()
+        _check(lib().tfg_engine_verlet_entries(self._h, ctypes.byref(m), ctypes.byref(x)))
+        return m.value, x.value
+
     def set_dpd_noise(self, mode, seed=0):
         _check(lib().tfg_engine_set_dpd_noise(self._h, mode, seed))
 
@@ -414,7 +422,7 @@ class Engine:
     def kernel_times(self):
         out = np.zeros(8)
         n = _check(lib().tfg_kernel_times(self._h, _dptr(out), 8))
-        names = ["bin", "nonbond", "advance", "bin_avg", "nonbond_avg", "advance_avg"]
+        names = ["bin", "build", "nonbond", "advance", "bin_avg", "build_avg", "nonbond_avg", "advance_avg"]
         return dict(zip(names, out[:n].tolist()))
 
     # ---- readback ----

@@ -68,6 +68,12 @@ struct SysDesc {
 constexpr int kListSlotBits = 25;
 constexpr int kListSlotMask = (1 << kListSlotBits) - 1;
 __host__ __device__ inline int list_code(int ox, int oy, int oz) { return ((ox + 2) * 5 + (oy + 2)) * 5 + (oz + 2); }
+// Entry k of particle i: tiles of 64 particles, entry-major inside a tile, so
+// that a wave of consecutive particles reads one entry index as 256
+// contiguous bytes and one tile's entries stay within lcap * 256 bytes.
+__host__ __device__ inline size_t list_index(int k, int i, int lcap) {
+    return (static_cast<size_t>(i >> 6) * lcap + k) * 64 + (i & 63);
+}
 // Pair descriptors: vectors per type pair (the pair kernels' LDS table).
 constexpr int kPairVecs = 4;
 
@@ -147,7 +153,7 @@ template <typename real> struct KP {
     // Neighbour lists (null and 0 without them).
     int list_mode;
     int lcap;                         // entries per particle
-    int *list;                        // entry k of particle i at list[k * np + i]
+    int *list;                        // entry k of particle i at list[list_index(k, i, lcap)]
     int *lcount;                      // entries per particle at the build
     const int4 *lblock;               // list kernel blocks: system, first slot, count
     typename V<real>::v4 *xb;         // position in the frame of the build cell; w: type column bits

@@ -70,11 +70,12 @@ struct DeviceEnsemble {
     hipEvent_t ev[8];
     int nev = 0;
     float last_ms = 0.0f;
-    double kernel_ms[3] = {0, 0, 0};
+    // phases: bin, list build, pair pass, advance (events 0..4 of a step)
+    double kernel_ms[4] = {0, 0, 0, 0};
     hipGraphExec_t graph = nullptr;
     bool profile = false;
     std::vector<hipEvent_t> prof_ev;
-    double prof_ms[3] = {0, 0, 0};
+    double prof_ms[4] = {0, 0, 0, 0};
     int prof_steps = 0;
 
     SysDesc *d_sys = nullptr;

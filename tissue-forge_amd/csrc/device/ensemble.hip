@@ -142,10 +142,11 @@ template <typename real> static int enqueue_step(DeviceEnsemble *E, bool energy,
         if ((rc = run_bin<real>(E, Pe)) != TFG_OK) return rc;
         if (ev) TFG_HIP(hipEventRecord(ev[1], E->stream));
         if ((rc = run_list_build<real>(E, Pe)) != TFG_OK) return rc;
-        if ((rc = run_list_force<real>(E, Pe)) != TFG_OK) return rc;
         if (ev) TFG_HIP(hipEventRecord(ev[2], E->stream));
-        if ((rc = run_advance<real>(E, Pe, 1)) != TFG_OK) return rc;
+        if ((rc = run_list_force<real>(E, Pe)) != TFG_OK) return rc;
         if (ev) TFG_HIP(hipEventRecord(ev[3], E->stream));
+        if ((rc = run_advance<real>(E, Pe, 1)) != TFG_OK) return rc;
+        if (ev) TFG_HIP(hipEventRecord(ev[4], E->stream));
         if (energy && (rc = run_energy<real>(E, Pe)) != TFG_OK) return rc;
         return TFG_OK;
     }
@@ -158,6 +159,7 @@ template <typename real> static int enqueue_step(DeviceEnsemble *E, bool energy,
         if (ev && last) TFG_HIP(hipEventRecord(ev[0], E->stream));
         if ((rc = run_bin<real>(E, Q)) != TFG_OK) return rc;
         if (ev && last) TFG_HIP(hipEventRecord(ev[1], E->stream));
+        if (ev && last) TFG_HIP(hipEventRecord(ev[2], E->stream));   // no list build
         // The sub-step pair pass is the full pair kernel; only its pending
         // flux is kept (the forces are rewritten by the step's own pass).
         if (!rk4)
@@ -166,9 +168,9 @@ template <typename real> static int enqueue_step(DeviceEnsemble *E, bool energy,
                 if ((rc = run_flux_integrate<real>(E, P)) != TFG_OK) return rc;
             }
         if ((rc = run_nonbond<real>(E, Q)) != TFG_OK) return rc;
-        if (ev && last) TFG_HIP(hipEventRecord(ev[2], E->stream));
-        if ((rc = run_advance<real>(E, Q, rk4 ? 2 + st : 1)) != TFG_OK) return rc;
         if (ev && last) TFG_HIP(hipEventRecord(ev[3], E->stream));
+        if ((rc = run_advance<real>(E, Q, rk4 ? 2 + st : 1)) != TFG_OK) return rc;
+        if (ev && last) TFG_HIP(hipEventRecord(ev[4], E->stream));
     }
     if (energy && (rc = run_energy<real>(E, Pe)) != TFG_OK) return rc;
     return TFG_OK;
@@ -305,13 +307,21 @@ template <typename real> static int upload(DeviceEnsemble *E) {
             }
             d.span_z_b = sb[2];
             d.stencil_off_b = static_cast<int>(stencil.size() / 2);
+            // Each column keeps the z cells whose closest point is within
+            // cutoff + skin (the full cell cull); its z span rides above the
+            // low byte of the y offset.
             for (int ox = -sb[0]; ox <= sb[0]; ox++)
                 for (int oy = -sb[1]; oy <= sb[1]; oy++) {
                     const double lx = e.geom.h[0] * std::fmax(std::abs(ox) - 1, 0);
                     const double ly = e.geom.h[1] * std::fmax(std::abs(oy) - 1, 0);
                     if (lx * lx + ly * ly > rcs * rcs) continue;
+                    int zs = 0;
+                    for (int oz = 1; oz <= sb[2]; oz++) {
+                        const double lz = e.geom.h[2] * std::fmax(oz - 1, 0);
+                        if (lx * lx + ly * ly + lz * lz <= rcs * rcs) zs = oz;
+                    }
                     stencil.push_back(ox);
-                    stencil.push_back(oy);
+                    stencil.push_back((oy & 0xff) | (zs << 8));
                 }
             d.nstencil_b = static_cast<int>(stencil.size() / 2) - d.stencil_off_b;
             d.cutb2 = rcs * rcs * (1.0 + 1e-5);
@@ -447,6 +457,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
         for (int s = 0; s < nsys; s++)
             for (int f = 0; f < sys[s].np; f += 256)
                 lblocks.push_back(make_int4(s, sys[s].p_off + f, std::min(256, sys[s].np - f), 0));
+    while (lblocks.size() % 8 != 0) lblocks.push_back(make_int4(0, 0, 0, 0));   // empty: the XCD remap needs 8k
     E->nlblocks = static_cast<int>(lblocks.size());
 
     // Particles in id order (slot == global id) into buffer B, and the bonded
@@ -560,7 +571,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     ALLOC(E->d_lflags, int, 4);
     ALLOC(E->d_sflags, int, 3 * nsys);
     if (E->list_mode) {
-        ALLOC(E->d_list, int, (size_t)np * E->lcap);
+        ALLOC(E->d_list, int, (size_t)((np + 63) / 64) * 64 * E->lcap);
         ALLOC(E->d_lcount, int, np);
         ALLOC(E->d_cid0, int, np);
         ALLOC(E->d_lblock, int4, lblocks.size());
@@ -695,8 +706,9 @@ int ensemble_download_one(DeviceEnsemble *E, int sys) {
 
 template <typename real> static int step_t(DeviceEnsemble *E, int nsteps, bool use_graph) {
     int rc;
-    TFG_HIP(hipEventRecord(E->ev[4], E->stream));
-    if (use_graph && nsteps > 1) {
+    constexpr int kEv = 5;   // phase events per step
+    TFG_HIP(hipEventRecord(E->ev[5], E->stream));
+    if (use_graph && nsteps > 1 && !E->profile) {
         // One energy-free step is captured once and replayed; the last step
         // runs eagerly with the energy pass.
         if (!E->graph) {
@@ -716,33 +728,33 @@ template <typename real> static int step_t(DeviceEnsemble *E, int nsteps, bool u
         for (int k = 0; k < nsteps - 1; k++) TFG_HIP(hipGraphLaunch(E->graph, E->stream));
         if ((rc = enqueue_step<real>(E, true, nullptr)) != TFG_OK) return rc;
     } else {
-        if (E->profile && static_cast<int>(E->prof_ev.size()) < 4 * nsteps) {
-            for (size_t k = E->prof_ev.size(); k < static_cast<size_t>(4 * nsteps); k++) {
+        if (E->profile && static_cast<int>(E->prof_ev.size()) < kEv * nsteps) {
+            for (size_t k = E->prof_ev.size(); k < static_cast<size_t>(kEv * nsteps); k++) {
                 hipEvent_t e;
                 TFG_HIP(hipEventCreate(&e));
                 E->prof_ev.push_back(e);
             }
         }
         for (int k = 0; k < nsteps; k++) {
-            hipEvent_t *ev = E->profile ? &E->prof_ev[4 * k] : (k == nsteps - 1 ? E->ev : nullptr);
+            hipEvent_t *ev = E->profile ? &E->prof_ev[kEv * k] : (k == nsteps - 1 ? E->ev : nullptr);
             if ((rc = enqueue_step<real>(E, k == nsteps - 1, ev)) != TFG_OK) return rc;
         }
     }
-    TFG_HIP(hipEventRecord(E->ev[5], E->stream));
+    TFG_HIP(hipEventRecord(E->ev[6], E->stream));
     TFG_HIP(hipStreamSynchronize(E->stream));
-    TFG_HIP(hipEventElapsedTime(&E->last_ms, E->ev[4], E->ev[5]));
-    if (!use_graph || nsteps <= 1) {
-        hipEvent_t *last = E->profile ? &E->prof_ev[4 * (nsteps - 1)] : E->ev;
-        for (int k = 0; k < 3; k++) {
+    TFG_HIP(hipEventElapsedTime(&E->last_ms, E->ev[5], E->ev[6]));
+    if (!use_graph || nsteps <= 1 || E->profile) {
+        hipEvent_t *last = E->profile ? &E->prof_ev[kEv * (nsteps - 1)] : E->ev;
+        for (int k = 0; k < kEv - 1; k++) {
             float ms = 0;
             TFG_HIP(hipEventElapsedTime(&ms, last[k], last[k + 1]));
             E->kernel_ms[k] = ms;
         }
         if (E->profile) {
-            for (int k = 0; k < 3; k++) E->prof_ms[k] = 0;
+            for (int k = 0; k < kEv - 1; k++) E->prof_ms[k] = 0;
             for (int st = 0; st < nsteps; st++) {
-                hipEvent_t *ev = &E->prof_ev[4 * st];
-                for (int k = 0; k < 3; k++) {
+                hipEvent_t *ev = &E->prof_ev[kEv * st];
+                for (int k = 0; k < kEv - 1; k++) {
                     float ms = 0;
                     TFG_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
                     E->prof_ms[k] += ms;
@@ -769,6 +781,25 @@ template <typename real> static int step_t(DeviceEnsemble *E, int nsteps, bool u
                                               " entries per particle) exceeded; raise it with tfg_engine_set_verlet");
         }
     }
+    return TFG_OK;
+}
+
+int ensemble_list_entries(DeviceEnsemble *E, int sys, double *mean, int *maxn) {
+    if (mean) *mean = 0;
+    if (maxn) *maxn = 0;
+    if (!E->list_mode || sys < 0 || sys >= E->nsys || E->sys[sys].np == 0) return TFG_OK;
+    const SysDesc &d = E->sys[sys];
+    std::vector<int> cnt(d.np);
+    TFG_HIP(hipStreamSynchronize(E->stream));
+    TFG_HIP(hipMemcpy(cnt.data(), E->d_lcount + d.p_off, sizeof(int) * d.np, hipMemcpyDeviceToHost));
+    long long sum = 0;
+    int mx = 0;
+    for (int c : cnt) {
+        sum += c;
+        mx = std::max(mx, c);
+    }
+    if (mean) *mean = static_cast<double>(sum) / d.np;
+    if (maxn) *maxn = mx;
     return TFG_OK;
 }
 
@@ -807,13 +838,14 @@ template <typename real> static int force_t(DeviceEnsemble *E) {
     TFG_HIP(hipEventRecord(E->ev[0], E->stream));
     if ((rc = run_bin<real>(E, P)) != TFG_OK) return rc;
     TFG_HIP(hipEventRecord(E->ev[1], E->stream));
-    if ((rc = run_nonbond<real>(E, P)) != TFG_OK) return rc;
     TFG_HIP(hipEventRecord(E->ev[2], E->stream));
-    if ((rc = run_advance<real>(E, P, 0)) != TFG_OK) return rc;
+    if ((rc = run_nonbond<real>(E, P)) != TFG_OK) return rc;
     TFG_HIP(hipEventRecord(E->ev[3], E->stream));
+    if ((rc = run_advance<real>(E, P, 0)) != TFG_OK) return rc;
+    TFG_HIP(hipEventRecord(E->ev[4], E->stream));
     if ((rc = run_energy<real>(E, P)) != TFG_OK) return rc;
     TFG_HIP(hipStreamSynchronize(E->stream));
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 4; k++) {
         float ms = 0;
         TFG_HIP(hipEventElapsedTime(&ms, E->ev[k], E->ev[k + 1]));
         E->kernel_ms[k] = ms;
@@ -864,7 +896,7 @@ int ensemble_autotune(DeviceEnsemble *E, int reps, int *blocks_out, int *stream_
         for (int r = 0; r < reps; r++) {
             const int rc = E->precision == PREC_MIXED ? force_t<float>(E) : force_t<double>(E);
             if (rc != TFG_OK) return -1.0;
-            const double t = which == 1 ? E->kernel_ms[1] : E->kernel_ms[0] + E->kernel_ms[2];
+            const double t = which == 1 ? E->kernel_ms[2] : E->kernel_ms[0] + E->kernel_ms[3];
             best = std::min(best, t);
         }
         return best;
@@ -927,13 +959,14 @@ int ensemble_set_launch(DeviceEnsemble *E, int blocks, int threads) {
 }
 
 int ensemble_timing(DeviceEnsemble *E, double *ms_out, int cap, const char **names_out) {
-    static const char *names[6] = {"bin", "nonbond", "advance", "bin_avg", "nonbond_avg", "advance_avg"};
-    const int n = std::min(cap, 6);
+    static const char *names[8] = {"bin",     "build",     "nonbond",     "advance",
+                                   "bin_avg", "build_avg", "nonbond_avg", "advance_avg"};
+    const int n = std::min(cap, 8);
     for (int k = 0; k < n; k++) {
-        if (k < 3)
+        if (k < 4)
             ms_out[k] = E->kernel_ms[k];
         else
-            ms_out[k] = E->prof_steps > 0 ? E->prof_ms[k - 3] / E->prof_steps : 0.0;
+            ms_out[k] = E->prof_steps > 0 ? E->prof_ms[k - 4] / E->prof_steps : 0.0;
         if (names_out) names_out[k] = names[k];
     }
     return n;

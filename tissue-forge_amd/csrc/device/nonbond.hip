@@ -469,13 +469,13 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                             while (m0) {
                                 const int t = __builtin_ctzll(m0);
                                 m0 &= m0 - 1ull;
-                                if (pos < P.lcap) P.list[(size_t)pos * P.np + si] = codes[t];
+                                if (pos < P.lcap) P.list[list_index(pos, si, P.lcap)] = codes[t];
                                 pos++;
                             }
                             while (m1) {
                                 const int t = __builtin_ctzll(m1);
                                 m1 &= m1 - 1ull;
-                                if (pos < P.lcap) P.list[(size_t)pos * P.np + si] = codes[64 + t];
+                                if (pos < P.lcap) P.list[list_index(pos, si, P.lcap)] = codes[64 + t];
                                 pos++;
                             }
                             lbase += last - b0;
@@ -558,7 +558,10 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                 const int st_off = BUILD ? sd.stencil_off_b : sd.stencil_off;
                 for (int col = 0; col < nst; col++) {
                     const int ox = P.stencil[2 * (st_off + col)];
-                    const int oy = P.stencil[2 * (st_off + col) + 1];
+                    const int oy_raw = P.stencil[2 * (st_off + col) + 1];
+                    // build columns carry their own z span above the low byte
+                    const int oy = BUILD ? static_cast<int>(static_cast<signed char>(oy_raw & 0xff)) : oy_raw;
+                    const int szc = BUILD ? (oy_raw >> 8) : sz;
                     int n0 = loc[0] + ox;
                     int n1 = loc[1] + oy;
                     // beyond a non-periodic face there is no neighbour
@@ -571,7 +574,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     n1 -= n1 >= sd.cdim[1] ? sd.cdim[1] : 0;
                     const int colbase = sd.cell_off + (n0 * sd.cdim[1] + n1) * sd.cdim[2];
                     const double shx = ox * sd.h[0], shy = oy * sd.h[1];
-                    int zlo = loc[2] - sz, zhi = loc[2] + sz;
+                    int zlo = loc[2] - szc, zhi = loc[2] + szc;
                     if (!(sd.periodic & 4)) {
                         zlo = max(zlo, 0);
                         zhi = min(zhi, sd.cdim[2] - 1);
@@ -585,10 +588,10 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                         if (!wrap_lo && !wrap_hi) {
                             zf = zlo; nzc = zhi - zlo + 1; oz0 = zlo - loc[2];
                         } else if (wrap_lo) {
-                            if (run == 0) { zf = zlo + sd.cdim[2]; nzc = -zlo; oz0 = -sz; }
+                            if (run == 0) { zf = zlo + sd.cdim[2]; nzc = -zlo; oz0 = -szc; }
                             else { zf = 0; nzc = zhi + 1; oz0 = -loc[2]; }
                         } else {
-                            if (run == 0) { zf = zlo; nzc = sd.cdim[2] - zlo; oz0 = -sz; }
+                            if (run == 0) { zf = zlo; nzc = sd.cdim[2] - zlo; oz0 = -szc; }
                             else { zf = 0; nzc = zhi - sd.cdim[2] + 1; oz0 = sd.cdim[2] - loc[2]; }
                         }
                         // first slot of each cell of the run (cells 1 .. nzc-1 set oz)
@@ -840,7 +843,12 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     constexpr bool kGuarded = std::is_same<real, float>::value;
     constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
     extern __shared__ __align__(16) unsigned char s_dyn[];
-    const int4 lb = P.lblock[blockIdx.x];
+    // Workgroups go to the 8 XCDs in turn: the workgroups of one XCD take
+    // consecutive particle blocks (the grid is a multiple of 8), so that the
+    // partner positions they gather meet in that XCD's L2.
+    const int nblk = gridDim.x;
+    const int b = (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8;
+    const int4 lb = P.lblock[b];
     const SysDesc &sd = P.sys[lb.x];
     const int nt = sd.ntypes;
     v4 *s_pair = reinterpret_cast<v4 *>(s_dyn);
@@ -873,12 +881,23 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     const int ti_row = ti * nt * kPairBytes;
     const int n = valid ? min(P.lcount[i], P.lcap) : 0;
     real fx = 0, fy = 0, fz = 0, nd = 0, ep = 0, vir = 0;
+    // Software pipeline: the entry two iterations ahead and the partner
+    // position one ahead are in flight while an iteration computes.
+    const int *lp = P.list + list_index(0, i, P.lcap);   // entry k at lp[64 k]
+    unsigned e_cur = n > 0 ? static_cast<unsigned>(lp[0]) : static_cast<unsigned>(i);
+    unsigned e_nxt = n > 1 ? static_cast<unsigned>(lp[64]) : static_cast<unsigned>(i);
+    v4 pj_cur = P.xb[e_cur & kListSlotMask];
     for (int k = 0; __any(k < n); k++) {
         const bool live = k < n;
-        const unsigned e = live ? static_cast<unsigned>(P.list[(size_t)k * P.np + i]) : static_cast<unsigned>(i);
+        const unsigned e = e_cur;
+        const unsigned e_after = k + 2 < n ? static_cast<unsigned>(lp[64 * (k + 2)]) : static_cast<unsigned>(i);
+        const v4 pj_nxt = P.xb[e_nxt & kListSlotMask];
         const int j = static_cast<int>(e & kListSlotMask);
         const v4 sh = s_sh[live ? (e >> kListSlotBits) : 62];
-        v4 pj = P.xb[j];
+        v4 pj = pj_cur;
+        e_cur = e_nxt;
+        e_nxt = e_after;
+        pj_cur = pj_nxt;
         pj.x = pj.x + sh.x;
         pj.y = pj.y + sh.y;
         pj.z = pj.z + sh.z;

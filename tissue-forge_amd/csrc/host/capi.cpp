@@ -186,6 +186,14 @@ int tfg_engine_verlet_state(const tfg_engine *h, int *active, int64_t *rebuilds,
     return TFG_OK;
 }
 
+int tfg_engine_verlet_entries(const tfg_engine *h, double *mean, int *maxn) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    if (mean) *mean = 0;
+    if (maxn) *maxn = 0;
+    if (!h->e.device) return TFG_OK;
+    TFG_GUARD(return ensemble_list_entries(h->e.device, h->e.device_index, mean, maxn));
+}
+
 int tfg_engine_set_nspecies(tfg_engine *h, int nspecies) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
     if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");
@@ -402,7 +410,8 @@ int tfg_engine_step(tfg_engine *h, int nsteps) {
         return set_error(TFG_E_STATE, "engine is not on the device: this library only steps on the GPU (call tfg_engine_to_device)");
     if (ensemble_size(h->e.device) != 1)
         return set_error(TFG_E_STATE, "engine belongs to an ensemble; use tfg_ensemble_step");
-    TFG_GUARD(return ensemble_step(h->e.device, nsteps, false));
+    // Several steps replay the captured step graph (eager while profiling).
+    TFG_GUARD(return ensemble_step(h->e.device, nsteps, true));
 }
 
 int tfg_nonbond_eval(tfg_engine *h) {

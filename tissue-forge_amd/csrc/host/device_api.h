@@ -45,5 +45,7 @@ int ensemble_size(DeviceEnsemble *ens);
 int ensemble_set_profile(DeviceEnsemble *ens, bool on);
 // Neighbour lists: in use, rebuilds of system sys so far, entries per particle.
 int ensemble_list_state(DeviceEnsemble *ens, int sys, int *active, long long *rebuilds, int *capacity);
+// Mean and largest list length of system sys at its last build.
+int ensemble_list_entries(DeviceEnsemble *ens, int sys, double *mean, int *maxn);
 
 }  // namespace tfg
