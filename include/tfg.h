@@ -125,6 +125,31 @@ int tfg_engine_verlet_entries(const tfg_engine *e, double *mean, int *maxn);
 int tfg_engine_set_nspecies(tfg_engine *e, int nspecies);
 int tfg_engine_set_dpd_noise(tfg_engine *e, int mode, uint64_t seed);
 int64_t tfg_engine_time(const tfg_engine *e);
+int tfg_engine_nspecies(const tfg_engine *e);
+
+/* ---- engine state checkpoint: io::FIO::toFile / fromFile / toString
+ *      (source/io/tfFIO.h:247-276).  The document is JSON: geometry as
+ *      requested, Universe settings (dt, flux sub-steps, integrator, distance
+ *      fraction, step counter, DPD noise, list settings), boundary faces and
+ *      wall potentials, types, every table with its coefficients, bindings,
+ *      fluxes, particles in id order (type, flags, cell id, position relative
+ *      to the cell origin, velocity, last force, inverse mass, radius),
+ *      species and pending flux, bonds, angles and the last energies.  Numbers
+ *      are written so that they read back to the same double: a save / load
+ *      round trip is exact. ---- */
+/* Writes the state to path.  An engine on the device is read back first and
+ * stays on the device. */
+int tfg_engine_save(tfg_engine *e, const char *path);
+/* The same document in buf (zero-terminated).  *len receives its length
+ * without the terminator; with buf NULL only *len is set.  A capacity below
+ * *len + 1 fails with TFG_E_RANGE and sets *len. */
+int tfg_engine_to_string(tfg_engine *e, char *buf, int64_t cap, int64_t *len);
+/* A new host engine from a saved document (not on the device: call
+ * tfg_engine_to_device to continue stepping).  A document that is not an
+ * engine state, or whose contents are inconsistent, fails with
+ * TFG_E_INVALIDARG and names what is wrong. */
+int tfg_engine_load(tfg_engine **out, const char *path);
+int tfg_engine_from_string(tfg_engine **out, const char *text, int64_t len);
 
 /* ---- particle types: engine_addtype (source/mdcore/src/tfEngine.cpp:1040-1052).
  *      Returns the new type id (>= 0) or an error. ---- */

@@ -79,6 +79,11 @@ _SIGNATURES = {
                                                ctypes.POINTER(ctypes.c_int)]),
     "tfg_engine_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64]),
     "tfg_engine_time": (ctypes.c_int64, [_vp]),
+    "tfg_engine_nspecies": (ctypes.c_int, [_vp]),
+    "tfg_engine_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "tfg_engine_to_string": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
+    "tfg_engine_load": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_char_p]),
+    "tfg_engine_from_string": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_int64]),
     "tfg_engine_addtype": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "tfg_potential_morse": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 6 + [ctypes.c_int]),
     "tfg_potential_lennard_jones_12_6": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 5),
@@ -378,6 +383,41 @@ class Engine:
     @property
     def time(self):
         return lib().tfg_engine_time(self._h)
+
+    # ---- state checkpoint (io::FIO toFile / fromFile / toString) ----
+    def save(self, path):
+        """Writes the engine state as JSON (read back from the device first
+        when on it; the engine stays there)."""
+        _check(lib().tfg_engine_save(self._h, os.fsencode(path)))
+
+    def to_json(self):
+        n = ctypes.c_int64()
+        _check(lib().tfg_engine_to_string(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _check(lib().tfg_engine_to_string(self._h, buf, n.value + 1, ctypes.byref(n)))
+        return buf.raw[:n.value].decode()
+
+    @classmethod
+    def _adopt(cls, handle):
+        eng = cls.__new__(cls)
+        eng._h = handle
+        eng._pots = []
+        eng.nspecies = _check(lib().tfg_engine_nspecies(handle))
+        return eng
+
+    @classmethod
+    def load(cls, path):
+        """A new host engine from a saved state (call to_device to step it)."""
+        h = _vp()
+        _check(lib().tfg_engine_load(ctypes.byref(h), os.fsencode(path)))
+        return cls._adopt(h)
+
+    @classmethod
+    def from_json(cls, text):
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        h = _vp()
+        _check(lib().tfg_engine_from_string(ctypes.byref(h), data, len(data)))
+        return cls._adopt(h)
 
     # ---- device toggle (cuda_config.engine.to_device / from_device) ----
     def to_device(self, precision=PREC_MIXED):

@@ -19,6 +19,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "host", "capi.cpp"),
     os.path.join(HERE, "csrc", "host", "engine.cpp"),
     os.path.join(HERE, "csrc", "host", "potential.cpp"),
+    os.path.join(HERE, "csrc", "host", "checkpoint.cpp"),
 ]
 OBJDIR = os.path.join(HERE, "_build")
 

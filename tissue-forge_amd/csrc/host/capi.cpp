@@ -2,6 +2,7 @@
 #include "../../../include/tfg.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
@@ -214,6 +215,88 @@ int tfg_engine_set_dpd_noise(tfg_engine *h, int mode, uint64_t seed) {
 }
 
 int64_t tfg_engine_time(const tfg_engine *h) { return h ? h->e.time : -1; }
+
+int tfg_engine_nspecies(const tfg_engine *h) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    return h->e.nspecies;
+}
+
+// ---- state checkpoint (io::FIO, source/io/tfFIO.h:247-276) ----
+
+static int current_state(tfg_engine *h, std::string *doc) {
+    if (h->e.device) {   // read back, stay on the device
+        int rc = ensemble_download_one(h->e.device, h->e.device_index);
+        if (rc != TFG_OK) return rc;
+    }
+    *doc = engine_to_json(h->e);
+    return TFG_OK;
+}
+
+static int save_state(tfg_engine *h, const char *path) {
+    std::string doc;
+    int rc = current_state(h, &doc);
+    if (rc != TFG_OK) return rc;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return set_error(TFG_E_FAIL, std::string("cannot open ") + path + " for writing");
+    const size_t nw = std::fwrite(doc.data(), 1, doc.size(), f);
+    const int cl = std::fclose(f);
+    if (nw != doc.size() || cl != 0) return set_error(TFG_E_FAIL, std::string("writing ") + path + " failed");
+    return TFG_OK;
+}
+
+static int load_state(tfg_engine **out, const char *text, size_t len) {
+    tfg_engine *h = new tfg_engine();
+    const int rc = engine_from_json(h->e, text, len);
+    if (rc != TFG_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return TFG_OK;
+}
+
+static int load_file(tfg_engine **out, const char *path) {
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return set_error(TFG_E_INVALIDARG, std::string("cannot open ") + path);
+    std::string text;
+    char buf[1 << 16];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+    const bool bad = std::ferror(f) != 0;
+    std::fclose(f);
+    if (bad) return set_error(TFG_E_FAIL, std::string("reading ") + path + " failed");
+    return load_state(out, text.data(), text.size());
+}
+
+int tfg_engine_save(tfg_engine *h, const char *path) {
+    if (!h || !path) return set_error(TFG_E_INVALIDARG, "null argument");
+    TFG_GUARD(return save_state(h, path));
+}
+
+int tfg_engine_to_string(tfg_engine *h, char *buf, int64_t cap, int64_t *len) {
+    if (!h || !len) return set_error(TFG_E_INVALIDARG, "null argument");
+    TFG_GUARD({
+        std::string doc;
+        int rc = current_state(h, &doc);
+        if (rc != TFG_OK) return rc;
+        *len = static_cast<int64_t>(doc.size());
+        if (!buf) return TFG_OK;
+        if (cap < *len + 1) return set_error(TFG_E_RANGE, "buffer too small for the engine state");
+        std::memcpy(buf, doc.data(), doc.size());
+        buf[doc.size()] = 0;
+    });
+    return TFG_OK;
+}
+
+int tfg_engine_load(tfg_engine **out, const char *path) {
+    if (!out || !path) return set_error(TFG_E_INVALIDARG, "null argument");
+    TFG_GUARD(return load_file(out, path));
+}
+
+int tfg_engine_from_string(tfg_engine **out, const char *text, int64_t len) {
+    if (!out || !text || len < 0) return set_error(TFG_E_INVALIDARG, "null argument");
+    TFG_GUARD(return load_state(out, text, static_cast<size_t>(len)));
+}
 
 int tfg_engine_addtype(tfg_engine *h, const char *name, double mass, double radius, int dynamics) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");

@@ -26,6 +26,7 @@ int make_geometry(Geometry &g, const double origin[3], const double dim[3],
     for (int k = 0; k < 3; k++) {
         g.origin[k] = origin[k];
         g.dim[k] = dim[k];
+        g.cells[k] = cells[k];
         const double L = (dim[k] - origin[k]) / cells[k];
         g.cdim[k] = static_cast<int>(std::floor(dim[k] / L));
         g.h[k] = g.dim[k] / g.cdim[k];

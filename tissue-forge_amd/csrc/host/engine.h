@@ -79,6 +79,7 @@ struct AngleRec { int32_t i, j, k; int pot; };
 struct Geometry {
     double origin[3] = {0, 0, 0};
     double dim[3] = {10, 10, 10};
+    int cells[3] = {1, 1, 1};   // as requested; cdim can be one less (floor quirk)
     int cdim[3] = {1, 1, 1};
     double h[3] = {10, 10, 10};
     double ih[3] = {0.1, 0.1, 0.1};
@@ -92,6 +93,12 @@ struct Geometry {
 
 int make_geometry(Geometry &g, const double origin[3], const double dim[3],
                   const int cells[3], double cutoff);
+
+class Engine;
+// Engine state as a JSON document and back (checkpoint.cpp).  engine_from_json
+// fills a fresh (default-constructed) engine.
+std::string engine_to_json(const Engine &e);
+int engine_from_json(Engine &e, const char *text, size_t len);
 
 // Per-particle host record, indexed by particle id.
 struct HostParticle {
