@@ -206,6 +206,9 @@ int tfg_engine_addangles(tfg_engine *e, const tfg_potential *p, int n, const int
  *      (source/cuda/tfEngineConfig.cpp:67-172). ---- */
 int tfg_device_count(void);
 int tfg_device_set(int ordinal);
+/* cuda::EngineConfig::getDevice (source/cuda/tfEngineConfig.cpp:42-45): the device
+ * that uploads from the calling thread use; -1 without a device. */
+int tfg_device_get(void);
 /* cuda::EngineConfig::clearDevice (source/cuda/tfEngineConfig.cpp:57-65): back to device 0. */
 int tfg_device_clear(void);
 /* Uploads the engine; a second call while on the device is a no-op. */

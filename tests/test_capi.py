@@ -141,3 +141,21 @@ def test_verlet_settings(tf):
     for skin, cap in ((-0.1, 0), (float("nan"), 0), (0.3, -1)):
         with pytest.raises(tf.TfgError):
             eng.set_verlet(skin, cap)
+
+
+def test_engine_config_mirror(tf, W):
+    """tf.cuda.EngineConfig's names over an Engine (wraps/python/cuda.py):
+    launch settings made off the device are kept for the next upload; the
+    device getter answers without a device."""
+    scene = W.cell_sorting(n=200, L=10.0, cells=3)
+    eng, _ = W.build_engine(scene)
+    cfg = tf.EngineConfig(eng)
+    assert not cfg.on_device()
+    cfg.set_threads(numThreads=128)
+    cfg.set_blocks(numBlocks=64)
+    assert (cfg._blocks, cfg._threads) == (64, 128)
+    assert cfg.get_device() == tf.get_device()
+    if tf.device_count() == 0:
+        assert cfg.get_device() == -1
+        with pytest.raises(tf.TfgError):
+            cfg.to_device()

@@ -150,3 +150,33 @@ def test_autotune_keeps_results(tf, W, gpu, precision):
     eng.from_device()
     for k in ("cid", "x", "v", "f"):
         assert np.array_equal(p[k], p_ref[k]), k
+
+
+def test_engine_config_names(tf, W, gpu):
+    """The reference's Python toggle (tf.cuda.EngineConfig, used as in
+    examples/py/cell_sorting_cuda.py:98-124): settings made off the device
+    apply at to_device, launch shapes leave results unchanged, and the device
+    cannot be switched while the engine is on it."""
+    scene = _scene(W)
+    ref, _ = W.build_engine(scene)
+    ref.to_device(1)
+    ref.step(5)
+    pr = ref.particles()
+    ref.from_device()
+    eng, _ = W.build_engine(scene)
+    cfg = tf.EngineConfig(eng, precision=tf.PREC_FP64)
+    cfg.set_threads(numThreads=128)
+    cfg.set_blocks(numBlocks=96)
+    cfg.to_device()
+    assert cfg.on_device() and cfg.get_device() == 0
+    with pytest.raises(tf.TfgError):
+        cfg.set_device(0)
+    eng.step(2)
+    cfg.set_blocks(numBlocks=0)
+    cfg.refresh_potentials()
+    eng.step(3)
+    pa = eng.particles()
+    cfg.from_device()
+    assert not cfg.on_device()
+    for k in ("cid", "x", "v", "f"):
+        assert np.array_equal(pa[k], pr[k]), k

@@ -45,6 +45,9 @@ FROZEN_Z = 1 << 5
 _lib = None
 
 
+E_STATE = -0x7FFF0001   # TFG_E_STATE (0x8000FFFF as a signed 32-bit code)
+
+
 class TfgError(RuntimeError):
     """Raised when a C-ABI call returns a failure code."""
 
@@ -80,6 +83,7 @@ _SIGNATURES = {
     "tfg_engine_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64]),
     "tfg_engine_time": (ctypes.c_int64, [_vp]),
     "tfg_engine_nspecies": (ctypes.c_int, [_vp]),
+    "tfg_device_get": (ctypes.c_int, []),
     "tfg_engine_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "tfg_engine_to_string": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "tfg_engine_load": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_char_p]),
@@ -177,6 +181,73 @@ def set_device(ordinal):
 def clear_device():
     """EngineConfig.clearDevice: back to device 0."""
     _check(lib().tfg_device_clear())
+
+
+def get_device():
+    """The device that uploads use (cuda.EngineConfig.getDevice); -1 without one."""
+    return lib().tfg_device_get()
+
+
+class EngineConfig:
+    """tf.cuda.EngineConfig (source/cuda/tfEngineConfig.h:43-170, exported to
+    Python by wraps/python/cuda.py) for one Engine: the device toggle and the
+    launch settings under the reference's names.  set_blocks / set_threads
+    made while off the device apply at the next to_device, as in the
+    reference.  precision: PREC_MIXED (default) or PREC_FP64."""
+
+    def __init__(self, engine, precision=None):
+        self._engine = engine
+        self.precision = PREC_MIXED if precision is None else precision
+        self._blocks = 0
+        self._threads = 0
+
+    def on_device(self):
+        return self._engine.on_device
+
+    def get_device(self):
+        return get_device()
+
+    def set_device(self, deviceId=0):
+        if self.on_device():
+            raise TfgError(E_STATE, "engine is on the device")   # tfEngineConfig.cpp:48-49
+        set_device(deviceId)
+
+    def clear_device(self):
+        if self.on_device():
+            raise TfgError(E_STATE, "engine is on the device")
+        clear_device()
+
+    def to_device(self):
+        self._engine.to_device(self.precision)
+        if self._blocks or self._threads:
+            self._engine.set_launch(self._blocks, self._threads)
+
+    def from_device(self):
+        self._engine.from_device()
+
+    def set_blocks(self, numBlocks, deviceId=-1):
+        """Workgroups of the pair kernel (0: one cell per wave)."""
+        self._blocks = int(numBlocks)
+        if self.on_device():
+            self._engine.set_launch(self._blocks, 0)
+
+    def set_threads(self, numThreads, deviceId=-1):
+        """Threads per workgroup of the streaming kernels (a multiple of 64)."""
+        self._threads = int(numThreads)
+        if self.on_device():
+            self._engine.set_launch(0, self._threads)
+
+    def refresh_potentials(self):
+        self._engine.refresh()
+
+    def refresh_fluxes(self):
+        self._engine.refresh()
+
+    def refresh_boundary_conditions(self):
+        self._engine.refresh()
+
+    def refresh(self):
+        self._engine.refresh()
 
 
 class Potential:

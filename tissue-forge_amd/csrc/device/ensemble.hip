@@ -190,6 +190,13 @@ int device_count() {
     return n;
 }
 
+int device_current() {
+    int n = 0, d = -1;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return -1;
+    if (hipGetDevice(&d) != hipSuccess) return -1;
+    return d;
+}
+
 template <typename real> static int upload(DeviceEnsemble *E);
 
 int ensemble_create(DeviceEnsemble **out, Engine *const *engines, int n, int precision) {

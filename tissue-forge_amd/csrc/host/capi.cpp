@@ -454,6 +454,7 @@ int tfg_engine_addangles(tfg_engine *h, const tfg_potential *p, int n, const int
 
 int tfg_device_count(void) { return device_count(); }
 int tfg_device_set(int ordinal) { return device_select(ordinal); }
+int tfg_device_get(void) { return device_current(); }
 int tfg_device_clear(void) { return device_count() > 0 ? device_select(0) : TFG_OK; }
 
 int tfg_engine_to_device(tfg_engine *h, int precision) {

@@ -14,6 +14,7 @@ struct DeviceEnsemble;
 enum Precision : int { PREC_MIXED = 0, PREC_FP64 = 1 };
 
 int device_select(int ordinal);
+int device_current();   // the calling thread's device, -1 without one
 int device_count();
 
 // Uploads the engines as independent systems of one ensemble.  Every engine
