@@ -56,8 +56,9 @@ struct SysDesc {
     int wall_off;
     int bc_kind[6];
     double bc_vel[6][3], bc_restore[6], bc_radius[6];
-    // Neighbour lists: the build stencil covers cutoff + skin; a particle
-    // displaced by more than skin / 2 since the build requests a rebuild.
+    // Neighbour lists: the build stencil (cells within cutoff + skin, as
+    // list codes) covers cutoff + skin; a particle displaced by more than
+    // skin / 2 since the build requests a rebuild.
     int nstencil_b, stencil_off_b, span_z_b;
     double cutb2;        // (cutoff + skin)^2, widened by a relative 1e-5
     double skin_half2;   // (skin / 2)^2, narrowed by a relative 1e-4
@@ -68,6 +69,8 @@ struct SysDesc {
 constexpr int kListSlotBits = 25;
 constexpr int kListSlotMask = (1 << kListSlotBits) - 1;
 __host__ __device__ inline int list_code(int ox, int oy, int oz) { return ((ox + 2) * 5 + (oy + 2)) * 5 + (oz + 2); }
+constexpr int kCentreCode = 62;        // list_code(0, 0, 0)
+constexpr int kMaxBuildCells = 128;    // build stencil cells (at most 5^3)
 // Entry k of particle i: tiles of 64 particles, entry-major inside a tile, so
 // that a wave of consecutive particles reads one entry index as 256
 // contiguous bytes and one tile's entries stay within lcap * 256 bytes.

@@ -314,22 +314,18 @@ template <typename real> static int upload(DeviceEnsemble *E) {
             }
             d.span_z_b = sb[2];
             d.stencil_off_b = static_cast<int>(stencil.size() / 2);
-            // Each column keeps the z cells whose closest point is within
-            // cutoff + skin (the full cell cull); its z span rides above the
-            // low byte of the y offset.
+            // The cells whose closest point is within cutoff + skin (the full
+            // cell cull), as list codes, one per stencil entry.
             for (int ox = -sb[0]; ox <= sb[0]; ox++)
-                for (int oy = -sb[1]; oy <= sb[1]; oy++) {
-                    const double lx = e.geom.h[0] * std::fmax(std::abs(ox) - 1, 0);
-                    const double ly = e.geom.h[1] * std::fmax(std::abs(oy) - 1, 0);
-                    if (lx * lx + ly * ly > rcs * rcs) continue;
-                    int zs = 0;
-                    for (int oz = 1; oz <= sb[2]; oz++) {
-                        const double lz = e.geom.h[2] * std::fmax(oz - 1, 0);
-                        if (lx * lx + ly * ly + lz * lz <= rcs * rcs) zs = oz;
+                for (int oy = -sb[1]; oy <= sb[1]; oy++)
+                    for (int oz = -sb[2]; oz <= sb[2]; oz++) {
+                        const double lx = e.geom.h[0] * std::fmax(std::abs(ox) - 1, 0);
+                        const double ly = e.geom.h[1] * std::fmax(std::abs(oy) - 1, 0);
+                        const double lz = e.geom.h[2] * std::fmax(std::abs(oz) - 1, 0);
+                        if (lx * lx + ly * ly + lz * lz > rcs * rcs) continue;
+                        stencil.push_back(list_code(ox, oy, oz));
+                        stencil.push_back(0);
                     }
-                    stencil.push_back(ox);
-                    stencil.push_back((oy & 0xff) | (zs << 8));
-                }
             d.nstencil_b = static_cast<int>(stencil.size() / 2) - d.stencil_off_b;
             d.cutb2 = rcs * rcs * (1.0 + 1e-5);
             const double half = 0.5 * e.verlet_skin * (1.0 - 1e-4);

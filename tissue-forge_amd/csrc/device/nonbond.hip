@@ -121,6 +121,7 @@ NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, b
         int b = align16((chunk + kPad) * static_cast<int>(sizeof(v4)));
         o->off_id = b;
         if (ids) b += align16(chunk * 4);
+        if (build) b += align16((kWave + 3 * kMaxBuildCells) * 4);   // build: self index, segment table
         o->off_slot = b;
         if (slots) b += align16(chunk * 4);
         o->off_vel = b;
@@ -208,6 +209,13 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
     unsigned char *region = s_dyn + P.nb.off_waves + wid * P.nb.region;
     v4 *s_pos = reinterpret_cast<v4 *>(region);
     int *s_id = reinterpret_cast<int *>(region + P.nb.off_id);
+    // BUILD (no ids there): the staged index of each i particle itself, then
+    // the segment table of the build stencil (ends, slot bases, offset codes)
+    int *s_self = reinterpret_cast<int *>(region + P.nb.off_id);
+    int *s_bend = s_self + kWave;
+    int *s_bbase = s_bend + kMaxBuildCells;
+    int *s_bcode = s_bbase + kMaxBuildCells;
+    (void)s_self; (void)s_bend; (void)s_bbase; (void)s_bcode;
     int *s_slot = reinterpret_cast<int *>(region + P.nb.off_slot);
     v4 *s_vel = reinterpret_cast<v4 *>(region + P.nb.off_vel);
     real *s_rad = reinterpret_cast<real *>(region + P.nb.off_rad);
@@ -254,6 +262,15 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
         const real w0 = real(sd.w_self);
         const real w2 = real(-1.5 * sd.w_self / sd.cutoff2);
         const real w3 = real(0.75 * sd.w_self / (sd.cutoff2 * sd.cutoff));
+        // BUILD: the i cell's box in its frame, and the squared reach of the
+        // list (cutoff + skin) widened by 1e-4 h for the rounding of stored
+        // positions: a candidate farther from the box than that cannot be
+        // within cutoff + skin of any particle of the cell.
+        const real box_lo[3] = {real(-sd.foff[0]), real(-sd.foff[1]), real(-sd.foff[2])};
+        const real box_hi[3] = {real(sd.h[0] - sd.foff[0]), real(sd.h[1] - sd.foff[1]), real(sd.h[2] - sd.foff[2])};
+        const double reach = sqrt(sd.cutb2) + 1e-4 * fmax(sd.h[0], fmax(sd.h[1], sd.h[2]));
+        const real cutbox2 = real(reach * reach);
+        (void)box_lo; (void)box_hi; (void)cutbox2;
         const long long tstep = sd.time0 + *P.step + P.step_ahead;
         const real dpd_delta = real(sd.inv_sqrt_dt);
         const real eps2 = eps_of<real>() * eps_of<real>();
@@ -264,6 +281,55 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
             if (ni == 0) continue;
             int loc[3];
             cell_loc(c - sd.cell_off, sd, loc);
+            // BUILD: the segment table of this cell's build stencil.  One lane
+            // per stencil cell loads its first slot and count (all in flight
+            // together); an inclusive scan gives each cell's end in the flat
+            // candidate order, so candidate t of the flat order is slot
+            // s_bbase[sg] + t of the cell sg with s_bend[sg - 1] <= t < s_bend[sg].
+            int btotal = 0;
+            if constexpr (BUILD) {
+                int carry = 0;
+#pragma unroll
+                for (int half = 0; half < kMaxBuildCells / kWave; half++) {
+                    const int sgi = half * kWave + lane;
+                    int cnt = 0, first = 0, code = kCentreCode;
+                    if (sgi < sd.nstencil_b) {
+                        code = P.stencil[2 * (sd.stencil_off_b + sgi)];
+                        const int o[3] = {code / 25 - 2, code / 5 % 5 - 2, code % 5 - 2};
+                        int nb[3];
+                        bool inside = true;
+#pragma unroll
+                        for (int k = 0; k < 3; k++) {
+                            nb[k] = loc[k] + o[k];
+                            // beyond a non-periodic face there is no neighbour
+                            if (sd.periodic & (1 << k)) {
+                                nb[k] += nb[k] < 0 ? sd.cdim[k] : 0;
+                                nb[k] -= nb[k] >= sd.cdim[k] ? sd.cdim[k] : 0;
+                            } else if (nb[k] < 0 || nb[k] >= sd.cdim[k]) {
+                                inside = false;
+                            }
+                        }
+                        if (inside) {
+                            const int cc = sd.cell_off + (nb[0] * sd.cdim[1] + nb[1]) * sd.cdim[2] + nb[2];
+                            first = P.start[cc];
+                            cnt = P.start[cc + 1] - first;
+                        }
+                    }
+                    int inc = cnt;
+#pragma unroll
+                    for (int o = 1; o < kWave; o <<= 1) {
+                        const int y = __shfl_up(inc, o);
+                        if (lane >= o) inc += y;
+                    }
+                    const int end = carry + inc;
+                    s_bend[sgi] = end;
+                    s_bbase[sgi] = first - (end - cnt);
+                    s_bcode[sgi] = code;
+                    carry += __shfl(inc, kWave - 1);
+                }
+                btotal = carry;
+                wave_sync();
+            }
 
             for (int ib = 0; ib < ni; ib += kWave) {
                 const int nib = min(kWave, ni - ib);
@@ -290,6 +356,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     for (int k = 0; k < S; k++) qacc[k] = 0;
                 int self_k = -1;   // staged index of the i particle itself in the current chunk
                 int lbase = 0;     // BUILD: list entries of this i particle so far
+                if (BUILD) s_self[lane] = -1;
 
                 // One candidate k of the staged list; ok = false makes it a no-op
                 // (k is then 0, a staged entry, so every read stays in range).
@@ -553,98 +620,158 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                 };
 
                 int nstaged = 0;
-                const int sz = BUILD ? sd.span_z_b : sd.span_z;
-                const int nst = BUILD ? sd.nstencil_b : sd.nstencil;
-                const int st_off = BUILD ? sd.stencil_off_b : sd.stencil_off;
-                for (int col = 0; col < nst; col++) {
-                    const int ox = P.stencil[2 * (st_off + col)];
-                    const int oy_raw = P.stencil[2 * (st_off + col) + 1];
-                    // build columns carry their own z span above the low byte
-                    const int oy = BUILD ? static_cast<int>(static_cast<signed char>(oy_raw & 0xff)) : oy_raw;
-                    const int szc = BUILD ? (oy_raw >> 8) : sz;
-                    int n0 = loc[0] + ox;
-                    int n1 = loc[1] + oy;
-                    // beyond a non-periodic face there is no neighbour
-                    // (space_init's task list skips it, tfSpace.cpp:913-944)
-                    if (!(sd.periodic & 1) && (n0 < 0 || n0 >= sd.cdim[0])) continue;
-                    if (!(sd.periodic & 2) && (n1 < 0 || n1 >= sd.cdim[1])) continue;
-                    n0 += n0 < 0 ? sd.cdim[0] : 0;
-                    n0 -= n0 >= sd.cdim[0] ? sd.cdim[0] : 0;
-                    n1 += n1 < 0 ? sd.cdim[1] : 0;
-                    n1 -= n1 >= sd.cdim[1] ? sd.cdim[1] : 0;
-                    const int colbase = sd.cell_off + (n0 * sd.cdim[1] + n1) * sd.cdim[2];
-                    const double shx = ox * sd.h[0], shy = oy * sd.h[1];
-                    int zlo = loc[2] - szc, zhi = loc[2] + szc;
-                    if (!(sd.periodic & 4)) {
-                        zlo = max(zlo, 0);
-                        zhi = min(zhi, sd.cdim[2] - 1);
-                    }
-                    // The column's cells are contiguous in memory unless the z range
-                    // wraps around the periodic boundary; then it is two runs.
-                    const bool wrap_lo = zlo < 0, wrap_hi = zhi >= sd.cdim[2];
-                    const int nruns = (wrap_lo || wrap_hi) ? 2 : 1;
-                    for (int run = 0; run < nruns; run++) {
-                        int zf, nzc, oz0;
-                        if (!wrap_lo && !wrap_hi) {
-                            zf = zlo; nzc = zhi - zlo + 1; oz0 = zlo - loc[2];
-                        } else if (wrap_lo) {
-                            if (run == 0) { zf = zlo + sd.cdim[2]; nzc = -zlo; oz0 = -szc; }
-                            else { zf = 0; nzc = zhi + 1; oz0 = -loc[2]; }
-                        } else {
-                            if (run == 0) { zf = zlo; nzc = sd.cdim[2] - zlo; oz0 = -szc; }
-                            else { zf = 0; nzc = zhi - sd.cdim[2] + 1; oz0 = sd.cdim[2] - loc[2]; }
+                if constexpr (BUILD) {
+                    // One pass over the build stencil's candidates in flat order
+                    // (the segment table below), 64 per pass, the next pass's
+                    // loads in flight while the current one is tested.  Only
+                    // candidates within reach of the i cell's box are staged,
+                    // packed in candidate order (wave ballot).
+                    const int ibase = i0 + ib;
+                    auto locate = [&](int t, int &sg) {
+                        while (t >= s_bend[sg]) sg++;
+                        return s_bbase[sg] + t;
+                    };
+                    bool in_c = lane < btotal;
+                    int sg_c = 0, sj_c = 0;
+                    if (in_c) sj_c = locate(lane, sg_c);
+                    v4 p_c = P.A.pos[sj_c];
+                    int m_c = P.A.meta[sj_c].x;
+                    int code_c = s_bcode[sg_c];
+                    for (int t0 = 0; t0 < btotal; t0 += kWave) {
+                        const int tn = t0 + kWave + lane;
+                        const bool in_n = tn < btotal;
+                        int sg_n = __shfl(sg_c, kWave - 1);
+                        int sj_n = 0;
+                        if (in_n) sj_n = locate(tn, sg_n);
+                        const v4 p_n = P.A.pos[sj_n];
+                        const int m_n = P.A.meta[sj_n].x;
+                        const int code_n = s_bcode[sg_n];
+
+                        const int ox = code_c / 25 - 2, oy = code_c / 5 % 5 - 2, oz = code_c % 5 - 2;
+                        v4 p = p_c;
+                        p.x = real(static_cast<double>(p.x) + ox * sd.h[0]);
+                        p.y = real(static_cast<double>(p.y) + oy * sd.h[1]);
+                        p.z = real(static_cast<double>(p.z) + oz * sd.h[2]);
+                        const real ex = fmax(fmax(box_lo[0] - p.x, p.x - box_hi[0]), real(0));
+                        const real ey = fmax(fmax(box_lo[1] - p.y, p.y - box_hi[1]), real(0));
+                        const real ez = fmax(fmax(box_lo[2] - p.z, p.z - box_hi[2]), real(0));
+                        const bool keep = in_c && ex * ex + ey * ey + ez * ez <= cutbox2;
+                        const unsigned long long km = __ballot(keep);
+                        if (keep) {
+                            const int dst = nstaged + __popcll(km & ((1ull << lane) - 1ull));
+                            p.w = as_real_bits((m_c & 0xffff) * kPairBytes, real(0));
+                            s_pos[dst] = p;
+                            s_slot[dst] = static_cast<int>(static_cast<unsigned>(sj_c) |
+                                                           (static_cast<unsigned>(code_c) << kListSlotBits));
+                            if (code_c == kCentreCode && static_cast<unsigned>(sj_c - ibase) < static_cast<unsigned>(nib))
+                                s_self[sj_c - ibase] = dst;
                         }
-                        // first slot of each cell of the run (cells 1 .. nzc-1 set oz)
-                        int bnd[2 * kMaxSpanZ + 1];
-#pragma unroll
-                        for (int m = 0; m < 2 * kMaxSpanZ + 1; m++)
-                            bnd[m] = m <= nzc ? P.start[colbase + zf + m] : 0x7fffffff;
-                        int ja = bnd[0];
-                        const int jend = P.start[colbase + zf + nzc];
-                        while (ja < jend) {
-                            const int take = min(jend - ja, chunk - nstaged);
-                            for (int t = lane; t < take; t += kWave) {
-                                const int sj = ja + t;
-                                int oz = oz0;
-#pragma unroll
-                                for (int m = 1; m < 2 * kMaxSpanZ + 1; m++) {
-                                    if (m >= nzc) break;
-                                    oz += sj >= bnd[m] ? 1 : 0;
-                                }
-                                v4 p = P.A.pos[sj];
-                                const int2 mj = *reinterpret_cast<const int2 *>(&P.A.meta[sj]);
-                                if (RADIUS) s_rad[nstaged + t] = p.w;
-                                p.x = real(static_cast<double>(p.x) + shx);
-                                p.y = real(static_cast<double>(p.y) + shy);
-                                p.z = real(static_cast<double>(p.z) + oz * sd.h[2]);
-                                p.w = as_real_bits((mj.x & 0xffff) * kPairBytes, real(0));
-                                s_pos[nstaged + t] = p;
-                                if (kIds) s_id[nstaged + t] = mj.y;
-                                if (kGuarded) s_slot[nstaged + t] = sj;
-                                if (BUILD)
-                                    s_slot[nstaged + t] = static_cast<int>(static_cast<unsigned>(sj) |
-                                                                           (static_cast<unsigned>(list_code(ox, oy, oz))
-                                                                            << kListSlotBits));
-                                if (DPD) s_vel[nstaged + t] = P.A.vel[sj];
-                                if (FLUX)
-                                    for (int k = 0; k < S; k++)
-                                        s_sp[(nstaged + t) * S + k] = P.A.sp[(size_t)k * P.np + sj];
+                        nstaged += __popcll(km);
+                        if (chunk - nstaged < kWave) {   // less than one pass of room left
+                            wave_sync();
+                            self_k = active ? s_self[il] : -1;
+                            process(nstaged);
+                            wave_sync();
+                            nstaged = 0;
+                            self_k = -1;
+                            s_self[lane] = -1;
+                        }
+                        in_c = in_n;
+                        sg_c = sg_n;
+                        sj_c = sj_n;
+                        p_c = p_n;
+                        m_c = m_n;
+                        code_c = code_n;
+                    }
+                } else {
+                    const int sz = sd.span_z;
+                    const int nst = sd.nstencil;
+                    const int st_off = sd.stencil_off;
+                    for (int col = 0; col < nst; col++) {
+                        const int ox = P.stencil[2 * (st_off + col)];
+                        const int oy = P.stencil[2 * (st_off + col) + 1];
+                        const int szc = sz;
+                        int n0 = loc[0] + ox;
+                        int n1 = loc[1] + oy;
+                        // beyond a non-periodic face there is no neighbour
+                        // (space_init's task list skips it, tfSpace.cpp:913-944)
+                        if (!(sd.periodic & 1) && (n0 < 0 || n0 >= sd.cdim[0])) continue;
+                        if (!(sd.periodic & 2) && (n1 < 0 || n1 >= sd.cdim[1])) continue;
+                        n0 += n0 < 0 ? sd.cdim[0] : 0;
+                        n0 -= n0 >= sd.cdim[0] ? sd.cdim[0] : 0;
+                        n1 += n1 < 0 ? sd.cdim[1] : 0;
+                        n1 -= n1 >= sd.cdim[1] ? sd.cdim[1] : 0;
+                        const int colbase = sd.cell_off + (n0 * sd.cdim[1] + n1) * sd.cdim[2];
+                        const double shx = ox * sd.h[0], shy = oy * sd.h[1];
+                        int zlo = loc[2] - szc, zhi = loc[2] + szc;
+                        if (!(sd.periodic & 4)) {
+                            zlo = max(zlo, 0);
+                            zhi = min(zhi, sd.cdim[2] - 1);
+                        }
+                        // The column's cells are contiguous in memory unless the z range
+                        // wraps around the periodic boundary; then it is two runs.
+                        const bool wrap_lo = zlo < 0, wrap_hi = zhi >= sd.cdim[2];
+                        const int nruns = (wrap_lo || wrap_hi) ? 2 : 1;
+                        for (int run = 0; run < nruns; run++) {
+                            int zf, nzc, oz0;
+                            if (!wrap_lo && !wrap_hi) {
+                                zf = zlo; nzc = zhi - zlo + 1; oz0 = zlo - loc[2];
+                            } else if (wrap_lo) {
+                                if (run == 0) { zf = zlo + sd.cdim[2]; nzc = -zlo; oz0 = -szc; }
+                                else { zf = 0; nzc = zhi + 1; oz0 = -loc[2]; }
+                            } else {
+                                if (run == 0) { zf = zlo; nzc = sd.cdim[2] - zlo; oz0 = -szc; }
+                                else { zf = 0; nzc = zhi - sd.cdim[2] + 1; oz0 = sd.cdim[2] - loc[2]; }
                             }
-                            if (si >= ja && si < ja + take) self_k = nstaged + si - ja;
-                            nstaged += take;
-                            ja += take;
-                            if (nstaged == chunk) {
-                                wave_sync();
-                                process(nstaged);
-                                wave_sync();
-                                nstaged = 0;
-                                self_k = -1;
+                            // first slot of each cell of the run (cells 1 .. nzc-1 set oz)
+                            int bnd[2 * kMaxSpanZ + 1];
+#pragma unroll
+                            for (int m = 0; m < 2 * kMaxSpanZ + 1; m++)
+                                bnd[m] = m <= nzc ? P.start[colbase + zf + m] : 0x7fffffff;
+                            int ja = bnd[0];
+                            const int jend = P.start[colbase + zf + nzc];
+                            while (ja < jend) {
+                                const int take = min(jend - ja, chunk - nstaged);
+                                for (int t = lane; t < take; t += kWave) {
+                                    const int sj = ja + t;
+                                    int oz = oz0;
+#pragma unroll
+                                    for (int m = 1; m < 2 * kMaxSpanZ + 1; m++) {
+                                        if (m >= nzc) break;
+                                        oz += sj >= bnd[m] ? 1 : 0;
+                                    }
+                                    v4 p = P.A.pos[sj];
+                                    const int2 mj = *reinterpret_cast<const int2 *>(&P.A.meta[sj]);
+                                    if (RADIUS) s_rad[nstaged + t] = p.w;
+                                    p.x = real(static_cast<double>(p.x) + shx);
+                                    p.y = real(static_cast<double>(p.y) + shy);
+                                    p.z = real(static_cast<double>(p.z) + oz * sd.h[2]);
+                                    p.w = as_real_bits((mj.x & 0xffff) * kPairBytes, real(0));
+                                    s_pos[nstaged + t] = p;
+                                    if (kIds) s_id[nstaged + t] = mj.y;
+                                    if (kGuarded) s_slot[nstaged + t] = sj;
+                                    if (DPD) s_vel[nstaged + t] = P.A.vel[sj];
+                                    if (FLUX)
+                                        for (int k = 0; k < S; k++)
+                                            s_sp[(nstaged + t) * S + k] = P.A.sp[(size_t)k * P.np + sj];
+                                }
+                                if (si >= ja && si < ja + take) self_k = nstaged + si - ja;
+                                nstaged += take;
+                                ja += take;
+                                if (nstaged == chunk) {
+                                    wave_sync();
+                                    process(nstaged);
+                                    wave_sync();
+                                    nstaged = 0;
+                                    self_k = -1;
+                                }
                             }
                         }
                     }
                 }
                 if (nstaged > 0) {
                     wave_sync();
+                    if (BUILD) self_k = active ? s_self[il] : -1;
                     process(nstaged);
                     wave_sync();
                 }
@@ -837,6 +964,8 @@ __device__ inline double list_guard_r2(const KP<double> &, const SysDesc &, int,
 // 125 build-cell offsets in LDS.  The pair test, the FP32 cutoff guard
 // (band pairs decided in FP64 from the stored positions), the number density
 // and the table evaluation are those of the cell kernel.
+constexpr int kListBatch = 4;   // list entries loaded together per lane (4 measured fastest of 4, 8, 16)
+
 template <typename real, bool ENERGY>
 __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     using v4 = typename V<real>::v4;
@@ -881,70 +1010,74 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     const int ti_row = ti * nt * kPairBytes;
     const int n = valid ? min(P.lcount[i], P.lcap) : 0;
     real fx = 0, fy = 0, fz = 0, nd = 0, ep = 0, vir = 0;
-    // Software pipeline: the entry two iterations ahead and the partner
-    // position one ahead are in flight while an iteration computes.
+    // Entries in batches of kListBatch: the batch's entries are loaded
+    // together, then its partner positions are gathered together, then the
+    // pairs are evaluated in list order (so many loads are in flight per lane
+    // and the sum keeps its order).
     const int *lp = P.list + list_index(0, i, P.lcap);   // entry k at lp[64 k]
-    unsigned e_cur = n > 0 ? static_cast<unsigned>(lp[0]) : static_cast<unsigned>(i);
-    unsigned e_nxt = n > 1 ? static_cast<unsigned>(lp[64]) : static_cast<unsigned>(i);
-    v4 pj_cur = P.xb[e_cur & kListSlotMask];
-    for (int k = 0; __any(k < n); k++) {
-        const bool live = k < n;
-        const unsigned e = e_cur;
-        const unsigned e_after = k + 2 < n ? static_cast<unsigned>(lp[64 * (k + 2)]) : static_cast<unsigned>(i);
-        const v4 pj_nxt = P.xb[e_nxt & kListSlotMask];
-        const int j = static_cast<int>(e & kListSlotMask);
-        const v4 sh = s_sh[live ? (e >> kListSlotBits) : 62];
-        v4 pj = pj_cur;
-        e_cur = e_nxt;
-        e_nxt = e_after;
-        pj_cur = pj_nxt;
-        pj.x = pj.x + sh.x;
-        pj.y = pj.y + sh.y;
-        pj.z = pj.z + sh.z;
-        const real dx = pi.x - pj.x;
-        const real dy = pi.y - pj.y;
-        const real dz = pi.z - pj.z;
-        const real r2 = dist2(pi, pj);
-        bool ok = live && r2 <= cut2;
-        const real ir = rsq(fmax(r2, eps2));
-        const real r_raw = r2 * ir;
-        const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + ti_row + real_bits_as_int(pj.w));
-        const v4 q0 = qd[0];
-        const v4 q1 = qd[1];
-        bool in = ok && r_raw <= q0.y;
-        if (kGuarded) {
-            const int meta = real_bits_as_int(q1.w);
-            bool band = ok && r2 > cut2_lo;
-            if (binside) band = band || (ok && (meta & kMetaBInside) && fabs(r_raw - q0.y) <= real(2.0 * kGuard) * q0.y);
-            if (__builtin_expect(band, false)) {
-                const double r2d = list_guard_r2(P, sd, i, j);
-                const v4 q2 = qd[2];
-                const double bd = static_cast<double>(q2.y) + static_cast<double>(q2.z);
-                ok = r2d <= sd.cutoff2;
-                in = ok && r2d <= bd * bd;
+    for (int k0 = 0; __any(k0 < n); k0 += kListBatch) {
+        unsigned ev[kListBatch];
+        v4 pv[kListBatch];
+#pragma unroll
+        for (int u = 0; u < kListBatch; u++)
+            ev[u] = k0 + u < n ? static_cast<unsigned>(lp[64 * (k0 + u)]) : static_cast<unsigned>(i);
+#pragma unroll
+        for (int u = 0; u < kListBatch; u++) pv[u] = P.xb[ev[u] & kListSlotMask];
+#pragma unroll
+        for (int u = 0; u < kListBatch; u++) {
+            const bool live = k0 + u < n;
+            const unsigned e = ev[u];
+            const int j = static_cast<int>(e & kListSlotMask);
+            const v4 sh = s_sh[live ? (e >> kListSlotBits) : kCentreCode];
+            v4 pj = pv[u];
+            pj.x = pj.x + sh.x;
+            pj.y = pj.y + sh.y;
+            pj.z = pj.z + sh.z;
+            const real dx = pi.x - pj.x;
+            const real dy = pi.y - pj.y;
+            const real dz = pi.z - pj.z;
+            const real r2 = dist2(pi, pj);
+            bool ok = live && r2 <= cut2;
+            const real ir = rsq(fmax(r2, eps2));
+            const real r_raw = r2 * ir;
+            const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + ti_row + real_bits_as_int(pj.w));
+            const v4 q0 = qd[0];
+            const v4 q1 = qd[1];
+            bool in = ok && r_raw <= q0.y;
+            if (kGuarded) {
+                const int meta = real_bits_as_int(q1.w);
+                bool band = ok && r2 > cut2_lo;
+                if (binside) band = band || (ok && (meta & kMetaBInside) && fabs(r_raw - q0.y) <= real(2.0 * kGuard) * q0.y);
+                if (__builtin_expect(band, false)) {
+                    const double r2d = list_guard_r2(P, sd, i, j);
+                    const v4 q2 = qd[2];
+                    const double bd = static_cast<double>(q2.y) + static_cast<double>(q2.z);
+                    ok = r2d <= sd.cutoff2;
+                    in = ok && r2d <= bd * bd;
+                }
             }
-        }
-        nd += ok ? w0 + r2 * (w2 + r_raw * w3) : real(0);
-        const real r = fmax(r_raw, q0.x);
-        const int ind = static_cast<int>(clamp_real(q0.z + r * (q0.w + r * q1.x), real(1), q1.y));
-        const int tb = real_bits_as_int(q1.z);
-        const v4 *c4 = reinterpret_cast<const v4 *>(s_dyn + tb + ind * 8 * static_cast<int>(sizeof(real)));
-        const v4 c0 = c4[0];
-        const v4 c1 = c4[1];
-        const real x = (r - c0.x) * c0.y;
-        real ee = c0.z * x + c0.w;
-        real eff = c0.z;
-        eff = eff * x + ee; ee = ee * x + c1.x;
-        eff = eff * x + ee; ee = ee * x + c1.y;
-        eff = eff * x + ee; ee = ee * x + c1.z;
-        eff = eff * x + ee; ee = ee * x + c1.w;
-        const real f = in ? eff * c0.y * ir : real(0);
-        fx -= f * dx;
-        fy -= f * dy;
-        fz -= f * dz;
-        if (ENERGY) {
-            ep += in ? ee : real(0);
-            vir -= f * r2;
+            nd += ok ? w0 + r2 * (w2 + r_raw * w3) : real(0);
+            const real r = fmax(r_raw, q0.x);
+            const int ind = static_cast<int>(clamp_real(q0.z + r * (q0.w + r * q1.x), real(1), q1.y));
+            const int tb = real_bits_as_int(q1.z);
+            const v4 *c4 = reinterpret_cast<const v4 *>(s_dyn + tb + ind * 8 * static_cast<int>(sizeof(real)));
+            const v4 c0 = c4[0];
+            const v4 c1 = c4[1];
+            const real x = (r - c0.x) * c0.y;
+            real ee = c0.z * x + c0.w;
+            real eff = c0.z;
+            eff = eff * x + ee; ee = ee * x + c1.x;
+            eff = eff * x + ee; ee = ee * x + c1.y;
+            eff = eff * x + ee; ee = ee * x + c1.z;
+            eff = eff * x + ee; ee = ee * x + c1.w;
+            const real f = in ? eff * c0.y * ir : real(0);
+            fx -= f * dx;
+            fy -= f * dy;
+            fz -= f * dz;
+            if (ENERGY) {
+                ep += in ? ee : real(0);
+                vir -= f * r2;
+            }
         }
     }
     if (valid) {
