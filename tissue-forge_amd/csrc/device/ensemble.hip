@@ -841,6 +841,9 @@ template <typename real> static int force_t(DeviceEnsemble *E) {
     TFG_HIP(hipEventRecord(E->ev[0], E->stream));
     if ((rc = run_bin<real>(E, P)) != TFG_OK) return rc;
     TFG_HIP(hipEventRecord(E->ev[1], E->stream));
+    // list mode: the lists of the new order (timed for the autotuner, which
+    // tunes the build's grid); the next step rebuilds again
+    if (E->list_mode && (rc = run_list_build<real>(E, P)) != TFG_OK) return rc;
     TFG_HIP(hipEventRecord(E->ev[2], E->stream));
     if ((rc = run_nonbond<real>(E, P)) != TFG_OK) return rc;
     TFG_HIP(hipEventRecord(E->ev[3], E->stream));
@@ -899,7 +902,10 @@ int ensemble_autotune(DeviceEnsemble *E, int reps, int *blocks_out, int *stream_
         for (int r = 0; r < reps; r++) {
             const int rc = E->precision == PREC_MIXED ? force_t<float>(E) : force_t<double>(E);
             if (rc != TFG_OK) return -1.0;
-            const double t = which == 1 ? E->kernel_ms[2] : E->kernel_ms[0] + E->kernel_ms[3];
+            // the pair-kernel grid: in list mode it is the list build's grid
+            // (a force-only pass builds the lists), the list pass has its own
+            const double t = which == 1 ? (E->list_mode ? E->kernel_ms[1] : E->kernel_ms[2])
+                                        : E->kernel_ms[0] + E->kernel_ms[3];
             best = std::min(best, t);
         }
         return best;

@@ -117,6 +117,7 @@ NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, b
     L.waves = kNbWaves;
     L.off_coef = align16(max_nt * max_nt * kPairVecs * static_cast<int>(sizeof(v4)));
     L.off_waves = L.off_coef + align16(coef_lds_count * static_cast<int>(sizeof(real)));
+    if (build) L.off_waves += align16(125 * static_cast<int>(sizeof(v4)));   // shifts of the 125 build-cell offsets
     auto region = [&](int chunk, NbLayout *o) {
         int b = align16((chunk + kPad) * static_cast<int>(sizeof(v4)));
         o->off_id = b;
@@ -184,6 +185,8 @@ __device__ __forceinline__ void write_pair_descriptor(const KP<real> &P, const S
     s_pair[kPairVecs * t + 3] = q3;
 }
 
+constexpr int kStageBatch = 4;   // list build: staging passes whose loads are issued together
+
 // BUILD: the neighbour-list build.  The same staging and distance test over
 // the stencil of cutoff + skin; the set bits are written to the lists
 // (partner slot and build-cell offset, in candidate order) instead of being
@@ -249,6 +252,15 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
             for (int t = threadIdx.x; t < sd.nflux; t += kNbThreads) s_flux[t] = P.flux[sd.flux_first + t];
         if (CLDS)
             for (int t = threadIdx.x; t < sd.coef_count; t += kNbThreads) s_coef[t] = P.coef[sd.coef_begin + t];
+        if (BUILD)   // shift of each build-cell offset code, in the working precision
+            for (int t = threadIdx.x; t < 125; t += kNbThreads) {
+                v4 sh;
+                sh.x = real((t / 25 - 2) * sd.h[0]);
+                sh.y = real((t / 5 % 5 - 2) * sd.h[1]);
+                sh.z = real((t % 5 - 2) * sd.h[2]);
+                sh.w = real(0);
+                reinterpret_cast<v4 *>(s_dyn + P.nb.off_coef)[t] = sh;
+            }
         __syncthreads();
 
         // FP32: the test accepts up to rc^2 (1 + kGuard); pairs above rc^2 (1 - kGuard)
@@ -533,16 +545,20 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                             const int b0 = seg0 > 0 ? before : 0;
                             int pos = lbase + inc - c - b0;
                             const int *codes = s_slot + kb;
+                            // entry pos of particle si, then 64 ints further per entry
+                            int *lw = P.list + list_index(pos, si, P.lcap);
                             while (m0) {
                                 const int t = __builtin_ctzll(m0);
                                 m0 &= m0 - 1ull;
-                                if (pos < P.lcap) P.list[list_index(pos, si, P.lcap)] = codes[t];
+                                if (pos < P.lcap) *lw = codes[t];
+                                lw += 64;
                                 pos++;
                             }
                             while (m1) {
                                 const int t = __builtin_ctzll(m1);
                                 m1 &= m1 - 1ull;
-                                if (pos < P.lcap) P.list[list_index(pos, si, P.lcap)] = codes[64 + t];
+                                if (pos < P.lcap) *lw = codes[64 + t];
+                                lw += 64;
                                 pos++;
                             }
                             lbase += last - b0;
@@ -621,67 +637,73 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
 
                 int nstaged = 0;
                 if constexpr (BUILD) {
-                    // One pass over the build stencil's candidates in flat order
-                    // (the segment table below), 64 per pass, the next pass's
-                    // loads in flight while the current one is tested.  Only
-                    // candidates within reach of the i cell's box are staged,
-                    // packed in candidate order (wave ballot).
+                    // The build stencil's candidates in flat order (the segment
+                    // table above), kStageBatch passes of 64 at a time: the
+                    // passes' loads are issued together, then each pass is
+                    // tested and packed in order.  Only candidates within reach
+                    // of the i cell's box are staged, in candidate order (wave
+                    // ballot).  Each lane's segment cursor only moves forward.
                     const int ibase = i0 + ib;
-                    auto locate = [&](int t, int &sg) {
-                        while (t >= s_bend[sg]) sg++;
-                        return s_bbase[sg] + t;
-                    };
-                    bool in_c = lane < btotal;
-                    int sg_c = 0, sj_c = 0;
-                    if (in_c) sj_c = locate(lane, sg_c);
-                    v4 p_c = P.A.pos[sj_c];
-                    int m_c = P.A.meta[sj_c].x;
-                    int code_c = s_bcode[sg_c];
-                    for (int t0 = 0; t0 < btotal; t0 += kWave) {
-                        const int tn = t0 + kWave + lane;
-                        const bool in_n = tn < btotal;
-                        int sg_n = __shfl(sg_c, kWave - 1);
-                        int sj_n = 0;
-                        if (in_n) sj_n = locate(tn, sg_n);
-                        const v4 p_n = P.A.pos[sj_n];
-                        const int m_n = P.A.meta[sj_n].x;
-                        const int code_n = s_bcode[sg_n];
-
-                        const int ox = code_c / 25 - 2, oy = code_c / 5 % 5 - 2, oz = code_c % 5 - 2;
-                        v4 p = p_c;
-                        p.x = real(static_cast<double>(p.x) + ox * sd.h[0]);
-                        p.y = real(static_cast<double>(p.y) + oy * sd.h[1]);
-                        p.z = real(static_cast<double>(p.z) + oz * sd.h[2]);
-                        const real ex = fmax(fmax(box_lo[0] - p.x, p.x - box_hi[0]), real(0));
-                        const real ey = fmax(fmax(box_lo[1] - p.y, p.y - box_hi[1]), real(0));
-                        const real ez = fmax(fmax(box_lo[2] - p.z, p.z - box_hi[2]), real(0));
-                        const bool keep = in_c && ex * ex + ey * ey + ez * ez <= cutbox2;
-                        const unsigned long long km = __ballot(keep);
-                        if (keep) {
-                            const int dst = nstaged + __popcll(km & ((1ull << lane) - 1ull));
-                            p.w = as_real_bits((m_c & 0xffff) * kPairBytes, real(0));
-                            s_pos[dst] = p;
-                            s_slot[dst] = static_cast<int>(static_cast<unsigned>(sj_c) |
-                                                           (static_cast<unsigned>(code_c) << kListSlotBits));
-                            if (code_c == kCentreCode && static_cast<unsigned>(sj_c - ibase) < static_cast<unsigned>(nib))
-                                s_self[sj_c - ibase] = dst;
+                    int sg = 0;
+                    for (int t0 = 0; t0 < btotal; t0 += kWave * kStageBatch) {
+                        int sjv[kStageBatch], codev[kStageBatch], mv[kStageBatch];
+                        v4 pv[kStageBatch];
+#pragma unroll
+                        for (int u = 0; u < kStageBatch; u++) {
+                            const int t = t0 + u * kWave + lane;
+                            sjv[u] = 0;
+                            if (t < btotal) {
+                                while (t >= s_bend[sg]) sg++;
+                                sjv[u] = s_bbase[sg] + t;
+                            }
+                            codev[u] = s_bcode[sg];
                         }
-                        nstaged += __popcll(km);
-                        if (chunk - nstaged < kWave) {   // less than one pass of room left
-                            wave_sync();
-                            self_k = active ? s_self[il] : -1;
-                            process(nstaged);
-                            wave_sync();
-                            nstaged = 0;
-                            self_k = -1;
-                            s_self[lane] = -1;
+#pragma unroll
+                        for (int u = 0; u < kStageBatch; u++) {
+                            pv[u] = P.A.pos[sjv[u]];
+                            mv[u] = P.A.meta[sjv[u]].x;
                         }
-                        in_c = in_n;
-                        sg_c = sg_n;
-                        sj_c = sj_n;
-                        p_c = p_n;
-                        m_c = m_n;
-                        code_c = code_n;
+#pragma unroll
+                        for (int u = 0; u < kStageBatch; u++) {
+                            const int tu = t0 + u * kWave;
+                            if (tu >= btotal) break;
+                            const bool in_c = tu + lane < btotal;
+                            const int code_c = codev[u], sj_c = sjv[u];
+                            // the shift in the working precision: the list keeps a
+                            // margin far above its rounding (the cutoff + skin
+                            // reach is widened by 1e-5, the rebuild threshold
+                            // narrowed by 1e-4)
+                            const v4 sh = reinterpret_cast<const v4 *>(s_dyn + P.nb.off_coef)[code_c];
+                            v4 p = pv[u];
+                            p.x = p.x + sh.x;
+                            p.y = p.y + sh.y;
+                            p.z = p.z + sh.z;
+                            const real ex = fmax(fmax(box_lo[0] - p.x, p.x - box_hi[0]), real(0));
+                            const real ey = fmax(fmax(box_lo[1] - p.y, p.y - box_hi[1]), real(0));
+                            const real ez = fmax(fmax(box_lo[2] - p.z, p.z - box_hi[2]), real(0));
+                            const bool keep = in_c && ex * ex + ey * ey + ez * ez <= cutbox2;
+                            const unsigned long long km = __ballot(keep);
+                            if (keep) {
+                                const int dst = nstaged + __popcll(km & ((1ull << lane) - 1ull));
+                                p.w = as_real_bits((mv[u] & 0xffff) * kPairBytes, real(0));
+                                s_pos[dst] = p;
+                                s_slot[dst] = static_cast<int>(static_cast<unsigned>(sj_c) |
+                                                               (static_cast<unsigned>(code_c) << kListSlotBits));
+                                if (code_c == kCentreCode &&
+                                    static_cast<unsigned>(sj_c - ibase) < static_cast<unsigned>(nib))
+                                    s_self[sj_c - ibase] = dst;
+                            }
+                            nstaged += __popcll(km);
+                            if (chunk - nstaged < kWave) {   // less than one pass of room left
+                                wave_sync();
+                                self_k = active ? s_self[il] : -1;
+                                process(nstaged);
+                                wave_sync();
+                                nstaged = 0;
+                                self_k = -1;
+                                s_self[lane] = -1;
+                            }
+                        }
                     }
                 } else {
                     const int sz = sd.span_z;
