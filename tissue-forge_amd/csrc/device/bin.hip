@@ -202,7 +202,9 @@ __global__ void k_permute(KP<real> P) {
         P.A.vel[i] = P.B.vel[src];
         const int4 m = P.B.meta[src];
         P.A.meta[i] = m;
-        P.A.imass[i] = P.B.imass[src];
+        // A step that keeps its lists keeps every slot: the inverse mass in A
+        // and the id -> slot map are still those of the last reordering
+        if (rebuild) P.A.imass[i] = P.B.imass[src];
         for (int k = 0; k < S; k++) {
             P.A.sp[(size_t)k * P.np + i] = P.B.sp[(size_t)k * P.np + src];
             P.A.q[(size_t)k * P.np + i] = P.B.q[(size_t)k * P.np + src];
@@ -221,7 +223,7 @@ __global__ void k_permute(KP<real> P) {
             P.A.sv[i] = P.B.sv[src];
             P.A.vk[i] = P.B.vk[src];
         }
-        P.slot_of[m.y] = i;
+        if (rebuild) P.slot_of[m.y] = i;
     }
 }
 
