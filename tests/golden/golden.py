@@ -23,7 +23,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 STEPS = 10
 ENERGY_KEYS = ["epot", "epot_nonbond", "epot_bond", "epot_angle", "ekin", "virial"]
-CASES = ["cell_sorting_small", "morse_lj_small", "dpd_small", "bonded_small"]
+CASES = ["cell_sorting_small", "morse_lj_small", "dpd_small", "bonded_small",
+         "walled_small", "couette_small", "bonded_rk4_small", "dpd_fluxsteps_small", "morse_lj_lists_small"]
 
 
 def path(case):
@@ -40,6 +41,22 @@ def scene_for(W, case):
         return W.dpd_droplets(n=1536, noise="philox")
     if case == "bonded_small":
         return W.bonded_chains(nchains=16, beads=32)
+    if case == "walled_small":   # potential + free-slip walls on every face, reflection
+        return W.walled_morse(n=1000, L=10.0, cells=3)
+    if case == "couette_small":   # no-slip floor, moving velocity lid, DPD walls, Philox noise
+        return W.dpd_couette(n=1500, wall_dpd=True, noise="philox")
+    if case == "bonded_rk4_small":   # Runge-Kutta 4
+        sc = W.bonded_chains(nchains=16, beads=32)
+        sc["integrator"] = 1
+        return sc
+    if case == "dpd_fluxsteps_small":   # three flux sub-steps per step
+        sc = W.dpd_droplets(n=1536, noise="philox")
+        sc["nr_fluxsteps"] = 3
+        return sc
+    if case == "morse_lj_lists_small":   # neighbour lists (skin 0.3) on the device; same pair set
+        sc = W.morse_lj(n=3000)
+        sc["verlet_skin"] = 0.3
+        return sc
     raise KeyError(case)
 
 

@@ -1,7 +1,9 @@
 """Writes the golden fixtures with the FP64 oracle (single-threaded).
 
 Run from the repository root after building the library and the oracle:
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
+With case names only those fixtures are written (tables.npz always covers
+every case).
 """
 import os
 import sys
@@ -36,19 +38,24 @@ def main():
     load_package()
     from tissue_forge_amd import workloads as W
     O = load_oracle()
+    only = set(sys.argv[1:])
+    unknown = only - set(golden.CASES)
+    if unknown:
+        raise SystemExit("unknown cases: %s" % sorted(unknown))
     tables = {}
     for case in golden.CASES:
         scene = golden.scene_for(W, case)
         pots = [W.make_potential(p) for p in scene["pots"]]
-        force, step = oracle_results(O, scene, pots, golden.STEPS)
-        golden.save(case, scene, force, step)
+        if not only or case in only:
+            force, step = oracle_results(O, scene, pots, golden.STEPS)
+            golden.save(case, scene, force, step)
+            print(case, scene["x"].shape[0], "particles", os.path.getsize(golden.path(case)), "bytes")
         for k, pot in enumerate(pots):
             info = pot.info()
             key = "%s_pot%d" % (case, k)
             tables[key + "_info"] = np.array([info["kind"], info["flags"], info["n"], info["a"], info["b"],
                                              info["r0_plusone"]] + list(info["alpha"]), dtype=np.float64)
             tables[key + "_coef"] = np.asarray(pot.coeffs(), dtype=np.float64)
-        print(case, scene["x"].shape[0], "particles", os.path.getsize(golden.path(case)), "bytes")
     np.savez_compressed(os.path.join(HERE, "tables.npz"), **tables)
 
 

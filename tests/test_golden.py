@@ -4,7 +4,11 @@ tests/golden/make_golden.py with the single-threaded FP64 oracle).
 CPU: the table builder, the scene generator and the oracle still reproduce
 the vectors (a change in any of them shows up here first).
 GPU: both device precisions against the vectors, from the stored inputs, so
-these tests need neither the generator nor the oracle.
+these tests need neither the generator nor the oracle.  The cases cover the
+cell pass (cell sorting, Morse+LJ, DPD with Philox noise and flux, bonded
+chains), walls and reflection, velocity and DPD walls, Runge-Kutta 4, flux
+sub-steps, and the neighbour-list pass (its trajectory against the oracle's,
+which has no lists).
 
 Tolerances: the table builder and the oracle are compared at 1e-13 relative
 (the same code on the same image is bit-identical; the margin covers libm
@@ -31,6 +35,9 @@ def _max_rel(a, b, floor):
     if not fa.any():
         return 0.0
     return float(np.max(np.abs(a[fa] - b[fb])) / max(float(np.max(np.abs(b[fb]))), floor))
+
+
+WALL_CASES = ("walled_small", "couette_small")
 
 
 def force_err(fg, fo):
@@ -99,6 +106,14 @@ def test_device_force_matches_golden(tf, W, gpu, case, precision, tol_f):
     assert np.array_equal(p["cid"], force["cid"])
     err = force_err(p["f"], force["f"])
     print(case, "precision", precision, "err_rms", err)
+    if precision == 0 and case in WALL_CASES:
+        # Near an edge or corner the wall force (one distance array across
+        # faces, tf_boundary_eval.h:209-252) scales as 1/r of the nearest wall,
+        # so the FP32 rounding of a position a few 1e-3 from a wall moves it by
+        # ~1e-5 relative against the unrounded inputs stored here; the wall
+        # arithmetic itself is checked at the path's own FP32 positions to
+        # 1e-5 in test_gpu_boundaries.py.
+        tol_f = 1e-3
     assert err <= tol_f
     eo = force["energies"]
     assert abs(e["epot"] - eo[0]) <= (1e-11 if precision == 1 else 1e-6) * max(abs(eo[0]), 1.0)
@@ -113,6 +128,8 @@ def test_device_trajectory_matches_golden(tf, W, gpu, case):
     scene, _, step = golden.load(case)
     eng, _ = W.build_engine(scene)
     eng.to_device(1)
+    if scene.get("verlet_skin"):   # the lists case steps with neighbour lists
+        assert eng.verlet_state()[0]
     eng.step(golden.STEPS)
     p = eng.particles()
     e = eng.energies()
