@@ -4,7 +4,7 @@ Workload: the 2,000,000-particle Morse+LJ system (4 types, density 1,
 cutoff 2.5, tabulated potentials), FP32-mixed path.  One step = binning,
 nonbonded pass, bonded pass, forward Euler with migration, all on the GPU
 with the particles resident in HBM.  By default the pair pass uses neighbour
-lists with a 0.3 skin (rebuilt on the device when a particle has moved more
+lists with a 0.4 skin (rebuilt on the device when a particle has moved more
 than half the skin; the pair set of every step is unchanged); the cell pass
 is timed beside it.
 
@@ -190,7 +190,7 @@ def main():
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--ens-steps", type=int, default=50)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
-    ap.add_argument("--skin", type=float, default=0.3, help="neighbour-list skin (0: cell pass only)")
+    ap.add_argument("--skin", type=float, default=0.4, help="neighbour-list skin (0: cell pass only)")
     ap.add_argument("--no-cell", action="store_true", help="skip the cell-pass comparison run")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -1040,9 +1040,14 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     for (int k0 = 0; __any(k0 < n); k0 += kListBatch) {
         unsigned ev[kListBatch];
         v4 pv[kListBatch];
+        // entries past the list are read from inside the allocation (index
+        // clamped to the capacity) and replaced by the particle itself, so the
+        // loads need no branch and no gather leaves the buffers
 #pragma unroll
-        for (int u = 0; u < kListBatch; u++)
-            ev[u] = k0 + u < n ? static_cast<unsigned>(lp[64 * (k0 + u)]) : static_cast<unsigned>(i);
+        for (int u = 0; u < kListBatch; u++) {
+            const unsigned raw = static_cast<unsigned>(lp[64 * min(k0 + u, P.lcap - 1)]);
+            ev[u] = k0 + u < n ? raw : static_cast<unsigned>(i);
+        }
 #pragma unroll
         for (int u = 0; u < kListBatch; u++) pv[u] = P.xb[ev[u] & kListSlotMask];
 #pragma unroll
