@@ -165,7 +165,10 @@ __device__ void reflect(const SysDesc &sd, const int loc[3], real xv[3], real vv
 
 // RK4 selects the instance with the Runge-Kutta stages (update 2..5); the
 // other serves updates 0 and 1.
-template <typename real, bool RK4>
+// LEAN: forward Euler for ensembles without bonds, angles, active walls or
+// species (the general kernel's registers are sized for those; without them
+// the kernel runs at twice the occupancy).
+template <typename real, bool RK4, bool LEAN>
 __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
     using v4 = typename V<real>::v4;
     const int S = P.S;
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
             const real h[3] = {real(sd.h[0]), real(sd.h[1]), real(sd.h[2])};
 
             double eb = 0.0, ea = 0.0;
-            for (int q = P.inc_off[id]; q < P.inc_off[id + 1]; q++) {
+            for (int q = LEAN ? 0 : P.inc_off[id]; q < (LEAN ? 0 : P.inc_off[id + 1]); q++) {
                 const int ent = P.inc[q];
                 const int kind = ent >> 30;   // 0 bond, 1 angle
                 const int idx = ent & 0x3fffffff;
@@ -270,7 +273,7 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                 }
             }
             const int ti = info & 0xffff;
-            if (sd.bc_active) {
+            if (!LEAN && sd.bc_active) {
                 const double ew = wall_forces<real>(P, sd, ti, loc, p, v, &f);
                 if (P.energy) P.e_nb[i] += ew;
             }
@@ -315,7 +318,7 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                         // x + foff >= h, x + foff < 0 in the stored frame
                         delta[k] = (xv[k] >= real(sd.h[k] - sd.foff[k])) - (xv[k] < real(-sd.foff[k]));
                     }
-                    if ((delta[0] != 0 || delta[1] != 0 || delta[2] != 0) && sd.bc_active)
+                    if (!LEAN && (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) && sd.bc_active)
                         reflect<real>(sd, loc, xv, vv, delta);
                     if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
                         moved = true;
@@ -461,7 +464,7 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
             // Particles that changed cell skip this step's integration and keep
             // their pending flux (Fluxes_integrate runs over the particles left
             // in the cell, tf_engine_advance.cpp:430-436).
-            for (int k = 0; k < S; k++) {
+            for (int k = 0; k < (LEAN ? 0 : S); k++) {
                 const size_t o = (size_t)k * P.np + i;
                 real s = P.A.sp[o], qq = P.A.q[o];
                 if (!RK4 && update == 1 && !moved) {
@@ -579,9 +582,11 @@ template <typename real> int run_flux_integrate(DeviceEnsemble *E, const KP<real
 
 template <typename real> int run_advance(DeviceEnsemble *E, const KP<real> &P, int update) {
     if (update >= 2)
-        hipLaunchKernelGGL((k_advance<real, true>), dim3(E->stream_grid()), dim3(256), 0, E->stream, P, update);
+        hipLaunchKernelGGL((k_advance<real, true, false>), dim3(E->stream_grid()), dim3(256), 0, E->stream, P, update);
+    else if (E->lean_advance)
+        hipLaunchKernelGGL((k_advance<real, false, true>), dim3(E->stream_grid()), dim3(256), 0, E->stream, P, update);
     else
-        hipLaunchKernelGGL((k_advance<real, false>), dim3(E->stream_grid()), dim3(256), 0, E->stream, P, update);
+        hipLaunchKernelGGL((k_advance<real, false, false>), dim3(E->stream_grid()), dim3(256), 0, E->stream, P, update);
     return launch_error("advance");
 }
 

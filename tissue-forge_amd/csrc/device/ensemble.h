@@ -51,6 +51,7 @@ struct DeviceEnsemble {
     int nsys = 0, np = 0, ncells = 0, S = 0;
     int max_sys_blocks = 1;   // energy partial blocks per system
     bool has_dpd = false, has_flux = false, has_radius = false;
+    bool lean_advance = false;   // no bonds, angles, active walls or species: the lean update kernel
     int integrator = INTEG_FORWARD_EULER;   // shared by every system of the ensemble
     int nr_fluxsteps = 1;                   // shared as well
     int coef_lds_bytes = 0;   // table bytes cached in LDS by the pair kernel (0: read from global)

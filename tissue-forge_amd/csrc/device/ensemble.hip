@@ -507,6 +507,11 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     }
     for (int g = 0; g < np; g++) deg[g + 1] += deg[g];
     inc.assign(deg[np], 0);
+    // the lean update kernel serves ensembles without bonds, angles, species
+    // or active walls
+    E->lean_advance = deg[np] == 0 && E->S == 0;
+    for (const SysDesc &d : E->sys)
+        if (d.bc_active) E->lean_advance = false;
     {
         std::vector<int> fillp(deg.begin(), deg.end() - 1);
         for (size_t b = 0; b < hb.size(); b++) {
