@@ -145,3 +145,26 @@ def test_integrator_change_on_device(tf, W, O, gpu):
     orc.step(3, 1)
     compare_fp64(eng, orc, scene)
     eng.from_device()
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_lean_update_equals_full_update(tf, W, gpu, precision):
+    """Ensembles without bonds, angles, species or active walls step with the
+    lean forward-Euler kernel.  The same scene with one inert species (no
+    flux: it never changes and moves nothing) takes the general kernel; both
+    must give the same trajectory bit for bit."""
+    out = []
+    for nspecies in (0, 1):
+        scene = W.cell_sorting(n=4000, L=15.0, cells=4)
+        if nspecies:
+            scene["nspecies"] = 1
+            scene["species"] = np.zeros((scene["x"].shape[0], 1))
+        eng, _ = W.build_engine(scene)
+        eng.to_device(precision)
+        eng.step(20)
+        out.append((eng.particles(), eng.energies()))
+        eng.from_device()
+    (pa, ea), (pb, eb) = out
+    for k in ("cid", "x", "v", "f"):
+        assert np.array_equal(pa[k], pb[k]), k
+    assert ea["epot"] == eb["epot"] and ea["ekin"] == eb["ekin"]
