@@ -187,8 +187,10 @@ __device__ __forceinline__ void write_pair_descriptor(const KP<real> &P, const S
 
 constexpr int kStageBatch = 4;   // list build: staging passes whose loads are issued together
 
-// BUILD: the neighbour-list build.  The same staging and distance test over
-// the stencil of cutoff + skin; the set bits are written to the lists
+// BUILD: the neighbour-list build.  The candidates are the cells of the
+// cutoff + skin stencil, staged from a per-cell segment table in one flat
+// pass and kept only within reach of the i cell's box; the same distance
+// test then marks the pairs, and the set bits are written to the lists
 // (partner slot and build-cell offset, in candidate order) instead of being
 // evaluated.  It runs only in the steps that rebuild.
 template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY, bool BUILD = false>
@@ -978,6 +980,8 @@ __device__ inline double list_guard_r2(const KP<float> &P, const SysDesc &sd, in
 }
 __device__ inline double list_guard_r2(const KP<double> &, const SysDesc &, int, int) { return 0.0; }
 
+constexpr int kListBatch = 4;   // list entries loaded together per lane (4 measured fastest of 4, 8, 16)
+
 // Pair forces from the neighbour lists: one lane per particle, entries in
 // list order (the build's candidate order), so each particle's sum has a
 // fixed order.  Tables only (no DPD, flux or radius transforms: those
@@ -986,7 +990,6 @@ __device__ inline double list_guard_r2(const KP<double> &, const SysDesc &, int,
 // 125 build-cell offsets in LDS.  The pair test, the FP32 cutoff guard
 // (band pairs decided in FP64 from the stored positions), the number density
 // and the table evaluation are those of the cell kernel.
-constexpr int kListBatch = 4;   // list entries loaded together per lane (4 measured fastest of 4, 8, 16)
 
 template <typename real, bool ENERGY>
 __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
