@@ -1045,10 +1045,11 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
         v4 pv[kListBatch];
         // entries past the list are read from inside the allocation (index
         // clamped to the capacity) and replaced by the particle itself, so the
-        // loads need no branch and no gather leaves the buffers
+        // loads need no branch and no gather leaves the buffers; entries are
+        // read once per step (non-temporal: L2 stays with the gathers)
 #pragma unroll
         for (int u = 0; u < kListBatch; u++) {
-            const unsigned raw = static_cast<unsigned>(lp[64 * min(k0 + u, P.lcap - 1)]);
+            const unsigned raw = static_cast<unsigned>(__builtin_nontemporal_load(lp + 64 * min(k0 + u, P.lcap - 1)));
             ev[u] = k0 + u < n ? raw : static_cast<unsigned>(i);
         }
 #pragma unroll
