@@ -176,3 +176,34 @@ def test_saving_on_host_changes_nothing(tf, W):
     after = eng.particles()
     for k in before:
         assert np.array_equal(before[k], after[k])
+
+
+def test_corrupted_documents_fail_cleanly(tf, W):
+    """Truncated, byte-flipped and spliced documents either load or raise
+    TfgError; none may crash the process (300 seeded mutations)."""
+    scene = W.bonded_chains(nchains=16, beads=32)
+    scene["integrator"] = 1
+    eng, _ = W.build_engine(scene)
+    doc = eng.to_json().encode()
+    rng = np.random.default_rng(11)
+    outcomes = {"loaded": 0, "refused": 0}
+    for trial in range(300):
+        kind = trial % 3
+        b = bytearray(doc)
+        if kind == 0:
+            b = b[:int(rng.integers(0, len(b)))]
+        elif kind == 1:
+            for _ in range(int(rng.integers(1, 8))):
+                b[int(rng.integers(0, len(b)))] = int(rng.integers(0, 256))
+        else:
+            a0, a1 = sorted(rng.integers(0, len(b), size=2))
+            c0 = int(rng.integers(0, len(b)))
+            b = b[:c0] + b[a0:a1] + b[c0:]
+        try:
+            back = tf.Engine.from_json(bytes(b))
+            back.close()
+            outcomes["loaded"] += 1
+        except tf.TfgError:
+            outcomes["refused"] += 1
+    print(outcomes)
+    assert outcomes["refused"] > 200
