@@ -194,8 +194,8 @@ def main():
     ap.add_argument("--no-cell", action="store_true", help="skip the cell-pass comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has a single MI355X, multi-GPU runs are not part "
-              "of this build", file=sys.stderr)
+        print("bench.py runs on one GPU only (--gpus 1): the engine step is measured on a single MI355X; "
+              "sharding across GPUs is designed but not implemented", file=sys.stderr)
         sys.exit(2)
 
     if args.calibrate_copy:
