@@ -1,0 +1,55 @@
+"""Parity at the full size of the measured workload: 2,000,000 particles of
+Morse+LJ (4 types, density 1, cutoff 2.5, 50^3 cells) on the neighbour-list
+path with skin 0.4, the configuration bench.py times.
+
+The first step's pair pass (lists built from the input positions) against
+the FP64 oracle evaluated at the same positions, as the path holds them:
+mixed err_rms <= 1e-5 (force error bound of the FP32-mixed path), FP64 <=
+1e-10; number densities likewise; and the cell ids of the binned input equal
+the oracle's.  The oracle runs on up to 16 host threads (~6 s).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def err_rms(fg, fo):
+    d = np.linalg.norm(fg[:, :3] - fo[:, :3], axis=1)
+    return float(d.max() / math.sqrt(float(np.mean(np.sum(fo[:, :3] ** 2, axis=1)))))
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-5), (1, 1e-10)])
+def test_full_size_list_forces_match_oracle(tf, W, O, gpu, precision, tol):
+    scene = W.morse_lj(n=2_000_000)
+    scene["verlet_skin"] = 0.4
+    eng, pots = W.build_engine(scene)
+    eng.to_device(precision)
+    assert eng.verlet_state()[0]
+    xq = eng.positions()
+    eng.step(1)
+    pg = eng.particles()
+    eng.from_device()
+    sq = dict(scene)
+    sq["x"] = xq
+    orc = O.Oracle(sq, pots)
+    orc.force(max(1, min(16, os.cpu_count() or 1)))
+    po = orc.particles()
+    assert np.array_equal(po["cid"], eng_cid_at_input(tf, W, scene, precision))
+    e = err_rms(pg["f"], po["f"])
+    nd = float(np.max(np.abs(pg["f"][:, 3] - po["f"][:, 3])) / np.max(po["f"][:, 3]))
+    print("precision", precision, "err_rms", e, "number density", nd)
+    assert e <= tol
+    assert nd <= tol
+
+
+def eng_cid_at_input(tf, W, scene, precision):
+    """Cell ids of the input as binned by the device (no step taken)."""
+    eng, _ = W.build_engine(scene)
+    eng.to_device(precision)
+    cid = eng.particles()["cid"]
+    eng.from_device()
+    return cid
