@@ -256,7 +256,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-mixed (fp32 pair/integrator arithmetic, fp64 reductions)",
+        "dtype": "f32",
+        "precision": "FP32-mixed: FP32 pair and integrator arithmetic, FP64 cutoff-band decisions and reductions",
         "data": "synthetic (seeded uniform positions)",
         "config": {"workload": "morse_lj_2M: %d particles, 4 types, density 1, cutoff 2.5, Morse like / LJ unlike, "
                                "cells %s" % (n, scene["cells"]),
