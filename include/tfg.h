@@ -123,8 +123,11 @@ int tfg_engine_verlet_state(const tfg_engine *e, int *active, int64_t *rebuilds,
 int tfg_engine_verlet_entries(const tfg_engine *e, double *mean, int *maxn);
 /* Species count per particle (the state vector of source/state/, flat here). */
 int tfg_engine_set_nspecies(tfg_engine *e, int nspecies);
+/* DPD random term: TFG_DPD_NOISE_UNIT or TFG_DPD_NOISE_PHILOX with its key. */
 int tfg_engine_set_dpd_noise(tfg_engine *e, int mode, uint64_t seed);
+/* Step counter (engine->time, source/mdcore/include/tfEngine.h:183). */
 int64_t tfg_engine_time(const tfg_engine *e);
+/* Species count per particle, as set (for an engine made by tfg_engine_load). */
 int tfg_engine_nspecies(const tfg_engine *e);
 
 /* ---- engine state checkpoint: io::FIO::toFile / fromFile / toString
