@@ -172,6 +172,31 @@ int tfg_potential_harmonic_angle(tfg_potential **out, double k, double theta0, d
                                  double max, double tol);
 int tfg_potential_dpd(tfg_potential **out, double alpha, double gamma, double sigma,
                       double cutoff, int shifted);
+/* Potential::overlapping_sphere (source/mdcore/src/tfPotential.cpp:1224-1307,
+ * 3086-3102; reference defaults mu 1, kc 1, kh 0, r0 0, min 0.001, max 10,
+ * tol 0.001): a table in the scaled distance r / (ri + rj) (POTENTIAL_SCALED).
+ * Negative tol selects the default. */
+int tfg_potential_overlapping_sphere(tfg_potential **out, double mu, double kc, double kh, double r0,
+                                     double min, double max, double tol);
+/* Potential::glj (source/mdcore/src/tfPotential.cpp:2085-2185, 3047-3066;
+ * reference defaults m 3, n 2m, k 0, r0 1, min 0.05 r0, max 3 r0, tol 0.01,
+ * not shifted): shifted evaluates at r - (ri + rj) + r0 + 1, otherwise at the
+ * scaled distance r / (ri + rj).  Negative tol selects the default. */
+int tfg_potential_glj(tfg_potential **out, double e, double m, double n, double k, double r0, double min,
+                      double max, double tol, int shifted);
+/* A table potential from its own record: the fields of struct Potential that
+ * potential_eval_ex reads (source/mdcore/include/tfPotential.h:217-252) and
+ * that the reference's GPU path packs per potential
+ * (engine_cuda_build_pots_pack, source/mdcore/src/tfRunner_cuda.cu:391-550):
+ * flags (PotentialFlags, tfPotential.h:56-108; SHIFTED and SCALED select the
+ * distance transform), the range [a, b], r0_plusone, alpha[0..2], n
+ * intervals and the 8 (n + 1) coefficients c.  The table is evaluated as
+ * given, so a binding hands over the reference's own Potential bit for bit.
+ * Fails with TFG_E_INVALIDARG unless 1 <= n <= 3048, ncoef = 8 (n + 1), a < b
+ * and every number is finite (except the 8 coefficients of interval 0, the
+ * continuation left of a, which the evaluation never reaches for r >= a). */
+int tfg_potential_from_table(tfg_potential **out, uint32_t flags, double a, double b, double r0_plusone,
+                             const double alpha[3], int n, const double *c, int64_t ncoef);
 int tfg_potential_destroy(tfg_potential *p);
 /* Table description: kind, flags, n, a, b, r0_plusone, alpha[0..2]. */
 int tfg_potential_info(const tfg_potential *p, int *kind, uint32_t *flags, int *n,
@@ -227,8 +252,18 @@ int tfg_engine_from_device(tfg_engine *e);
  * (source/mdcore/src/tfEngine.cpp:1070-1073, 1093-1096, tfSpace.cpp:493-496). */
 int tfg_engine_refresh(tfg_engine *e);
 int tfg_engine_on_device(const tfg_engine *e);
-/* cuda::EngineConfig::setBlocks / setThreads (source/cuda/tfEngineConfig.cpp:123-160):
- * workgroups of the pair kernel, threads of the streaming kernels (0 = keep). */
+/* cuda::EngineConfig::setBlocks / setThreads (source/cuda/tfEngineConfig.cpp:98-124):
+ * workgroups of the pair kernel, threads per workgroup of the streaming
+ * kernels (a multiple of 64 up to 1024), applied at the next
+ * tfg_engine_to_device (0 keeps the default).  As in the reference they fail
+ * while the engine is on the device (TFG_E_STATE, the reference's
+ * TFCUDAERR_ondevice). */
+int tfg_engine_set_blocks(tfg_engine *e, int blocks);
+int tfg_engine_set_threads(tfg_engine *e, int threads);
+int tfg_engine_launch_config(const tfg_engine *e, int *blocks, int *threads);
+/* Launch shape of the device copy itself, while on the device (no reference
+ * counterpart: the reference sets the shape only before the upload).  The
+ * autotuner's results can be replaced with it; no shape changes a result bit. */
 int tfg_set_launch(tfg_engine *e, int blocks, int threads);
 /* Launch-shape autotuner (no reference counterpart: the reference takes
  * setBlocks / setThreads by hand).  Times reps force-only passes per
@@ -238,7 +273,15 @@ int tfg_set_launch(tfg_engine *e, int blocks, int threads);
 int tfg_autotune(tfg_engine *e, int reps, int *blocks, int *stream_blocks);
 
 /* ---- step: engine_step (source/mdcore/src/tfEngine.cpp:1378-1429), whole step on
- *      the device (binning, nonbonded, bonded, flux, forward Euler). ---- */
+ *      the device (binning, nonbonded, bonded, flux, forward Euler).  A step
+ *      that fails (a particle moving two cells or more: TFG_E_TOOFAST; a
+ *      neighbour list over its capacity: TFG_E_RANGE) ends the call there, as
+ *      Universe_Step stops at the first failing engine_step
+ *      (source/tfUniverse.cpp:368-370): later steps do nothing, the step
+ *      counter includes the failing step (engine_step increments time first,
+ *      tfEngine.cpp:1388), and the state is the one the failing step left
+ *      (after its update when a particle moved too fast, that of the step
+ *      before for a list overflow). ---- */
 int tfg_engine_step(tfg_engine *e, int nsteps);
 /* engine_force only (source/mdcore/src/tfEngine.cpp:1504-1557): binning, nonbonded
  * and bonded forces, no update.  The role of cuda::engine_nonbond_cuda

@@ -1212,6 +1212,39 @@ int or_add_particles(void *h, int n, const int *types, const double *x, const do
     return 0;
 }
 
+// The state engine_fromCUDA writes back into the host engine
+// (tfEngine.cpp:1164-1197, engine_cuda_unload_parts): per particle id its
+// cell, position relative to the cell origin, velocity, species and pending
+// flux (any pointer may be null), and the step counter.  Particles move to
+// their given cells in id order.
+int or_set_state(void *h, const int *cid, const double *x, const double *v, const double *species,
+                 const double *q, long long time) {
+    System *s = static_cast<System *>(h);
+    std::vector<Part> all(s->nparts);
+    for (int id = 0; id < s->nparts; id++) all[id] = s->part(id);
+    if (cid)
+        for (int id = 0; id < s->nparts; id++)
+            if (cid[id] < 0 || cid[id] >= s->ncells) return -1;
+    std::vector<int> cell_of(s->nparts);
+    for (int id = 0; id < s->nparts; id++) cell_of[id] = cid ? cid[id] : s->where[id].first;
+    for (Cell &c : s->cells) c.parts.clear();
+    for (int id = 0; id < s->nparts; id++) {
+        Part &p = all[id];
+        for (int k = 0; k < 3; k++) {
+            if (x) p.x[k] = x[3 * id + k];
+            if (v) p.v[k] = v[3 * id + k];
+        }
+        for (int k = 0; k < s->nspecies; k++) {
+            if (species) p.s[k] = species[(size_t)id * s->nspecies + k];
+            if (q) p.q[k] = q[(size_t)id * s->nspecies + k];
+        }
+        s->cells[cell_of[id]].parts.push_back(p);
+    }
+    s->time = time;
+    s->reindex();
+    return 0;
+}
+
 int or_set_flags(void *h, int n, const int *ids, const unsigned *flags) {
     System *s = static_cast<System *>(h);
     for (int q = 0; q < n; q++) s->part(ids[q]).flags = flags[q];

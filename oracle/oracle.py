@@ -31,6 +31,7 @@ _SIG = {
     "or_add_flux": (ctypes.c_int, [_vp] + [ctypes.c_int] * 5 + [ctypes.c_double] * 4),
     "or_add_particles": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp]),
     "or_set_flags": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _up]),
+    "or_set_state": (ctypes.c_int, [_vp, _ip, _dp, _dp, _dp, _dp, ctypes.c_longlong]),
     "or_add_bonds": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip]),
     "or_add_angles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip]),
     "or_set_dpd_noise": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_ulonglong]),
@@ -202,6 +203,26 @@ class Oracle:
             out["species"] = sp[:, :self.nspecies]
             out["species_q"] = q[:, :self.nspecies]
         return out
+
+    def set_state(self, state, time):
+        """Places every particle (by id) in its given cell at its given
+        cell-relative position with the given velocity (and species), and sets
+        the step counter: the state engine_fromCUDA hands back."""
+        cid = np.ascontiguousarray(state["cid"], dtype=np.int32)
+        x = np.ascontiguousarray(state["x"], dtype=np.float64)
+        v = np.ascontiguousarray(state["v"], dtype=np.float64)
+        sp = q = None
+        if self.nspecies:
+            sp = np.ascontiguousarray(state["species"], dtype=np.float64)
+            q = np.ascontiguousarray(state["species_q"], dtype=np.float64)
+        rc = lib().or_set_state(self._h, cid.ctypes.data_as(_ip), _d(x), _d(v), _d(sp) if sp is not None else None,
+                                _d(q) if q is not None else None, int(time))
+        if rc != 0:
+            raise ValueError("oracle rejected the state")
+
+    @property
+    def time(self):
+        return lib().or_time(self._h)
 
     def energies(self):
         out = np.zeros(6)

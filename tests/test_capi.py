@@ -153,7 +153,12 @@ def test_engine_config_mirror(tf, W):
     assert not cfg.on_device()
     cfg.set_threads(numThreads=128)
     cfg.set_blocks(numBlocks=64)
-    assert (cfg._blocks, cfg._threads) == (64, 128)
+    assert cfg.launch_config() == (64, 128)
+    with pytest.raises(tf.TfgError):
+        cfg.set_threads(numThreads=100)   # not a multiple of 64
+    with pytest.raises(tf.TfgError):
+        cfg.set_blocks(numBlocks=-1)
+    assert cfg.launch_config() == (64, 128)
     assert cfg.get_device() == tf.get_device()
     if tf.device_count() == 0:
         assert cfg.get_device() == -1

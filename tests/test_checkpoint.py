@@ -168,6 +168,55 @@ def test_malformed_documents_are_refused(tf, W):
         tf.Engine.load("/nonexistent/state.json")
 
 
+def _table_mutation(tf, W, change, match):
+    def mutate(d):
+        pot = d["potentials"][0]
+        change(pot)
+    _bad(tf, W, mutate, match)
+
+
+def test_tables_the_device_cannot_evaluate_are_refused(tf, W):
+    """A table the device would read out of bounds (n = 0: the interval
+    clamp [1, n] then reaches past the coefficients) or evaluate from
+    non-finite numbers is refused on load, naming the potential."""
+    def zero_intervals(p):
+        p["n"] = 0
+        p["c"] = p["c"][:8]
+    _table_mutation(tf, W, zero_intervals, "interval count")
+    _table_mutation(tf, W, lambda p: p.update(a=p["b"]), "a < b")
+    _table_mutation(tf, W, lambda p: p.update(b="inf"), "a < b")
+    _table_mutation(tf, W, lambda p: p.update(alpha=["nan", 0, 0, 0]), "interval map")
+    _table_mutation(tf, W, lambda p: p["c"].__setitem__(9, "nan"), "coefficients must be finite")
+    _table_mutation(tf, W, lambda p: p.update(r0_plusone="-inf"), "r0")
+
+
+def test_table_from_its_own_record(tf, W):
+    """tfg_potential_from_table: a table handed over as its record
+    evaluates exactly like the one it came from, and malformed records are
+    refused."""
+    src = tf.Potential.morse(d=3.0, a=5.0, r0=1.0, min=0.5, max=3.0, tol=1e-3, shifted=True)
+    info, c = src.info(), src.coeffs()
+    p = tf.Potential.from_table(info["flags"], info["a"], info["b"], info["r0_plusone"], info["alpha"], info["n"], c)
+    assert p.info() == info
+    assert np.array_equal(p.coeffs(), c)
+    for r, ri, rj in ((0.8, 0.4, 0.3), (1.5, 0.2, 0.2), (2.9, 0.5, 0.6)):
+        assert p(r, ri, rj) == src(r, ri, rj)
+    bad = [
+        dict(n=0, c=c[:8]),
+        dict(n=info["n"] + 1),                      # coefficient count does not match n
+        dict(a=info["b"]),                          # empty range
+        dict(alpha=[math.nan, 0.0, 0.0]),
+        dict(c=np.where(np.arange(c.size) == 12, math.inf, c)),
+        dict(flags=1 << 20),                        # not a PotentialFlags bit
+    ]
+    for change in bad:
+        args = dict(flags=info["flags"], a=info["a"], b=info["b"], r0_plusone=info["r0_plusone"],
+                    alpha=info["alpha"], n=info["n"], c=c)
+        args.update(change)
+        with pytest.raises(tf.TfgError):
+            tf.Potential.from_table(**args)
+
+
 def test_saving_on_host_changes_nothing(tf, W):
     scene = W.bonded_chains(nchains=16, beads=32)
     eng, _ = W.build_engine(scene)

@@ -155,8 +155,9 @@ def test_autotune_keeps_results(tf, W, gpu, precision):
 def test_engine_config_names(tf, W, gpu):
     """The reference's Python toggle (tf.cuda.EngineConfig, used as in
     examples/py/cell_sorting_cuda.py:98-124): settings made off the device
-    apply at to_device, launch shapes leave results unchanged, and the device
-    cannot be switched while the engine is on it."""
+    apply at to_device, launch shapes leave results unchanged, and neither the
+    device nor the launch settings can change while the engine is on it
+    (TFCUDAERR_ondevice, tfEngineConfig.cpp:48-49, 98-124)."""
     scene = _scene(W)
     ref, _ = W.build_engine(scene)
     ref.to_device(1)
@@ -171,8 +172,13 @@ def test_engine_config_names(tf, W, gpu):
     assert cfg.on_device() and cfg.get_device() == 0
     with pytest.raises(tf.TfgError):
         cfg.set_device(0)
+    assert tf.lib().tfg_set_launch(eng._h, 0, 0) == 0
     eng.step(2)
-    cfg.set_blocks(numBlocks=0)
+    for call in (lambda: cfg.set_blocks(numBlocks=0), lambda: cfg.set_threads(numThreads=256)):
+        with pytest.raises(tf.TfgError) as ei:
+            call()
+        assert ei.value.code == tf.E_STATE
+    assert cfg.launch_config() == (96, 128)
     cfg.refresh_potentials()
     eng.step(3)
     pa = eng.particles()
@@ -180,3 +186,36 @@ def test_engine_config_names(tf, W, gpu):
     assert not cfg.on_device()
     for k in ("cid", "x", "v", "f"):
         assert np.array_equal(pa[k], pr[k]), k
+
+
+def test_continuing_on_cpu_after_from_device(tf, W, O, gpu):
+    """engine_fromCUDA hands the state back to the CPU engine
+    (tfEngine.cpp:1164-1197), which continues stepping.  The oracle (the
+    CPU restatement of engine_step) takes the state the device handed back
+    (cells, cell-relative positions, velocities, species, pending flux, step
+    counter) and continues; it must follow the device's own continuation and
+    the oracle's uninterrupted run (DPD with Philox noise keyed by the step
+    counter, Fick flux: the counter and the species travel too)."""
+    scene = W.dpd_droplets(n=6000, noise="philox")
+    eng, pots = W.build_engine(scene)
+    eng.to_device(1)
+    eng.step(5)
+    eng.from_device()
+    assert eng.time == 5
+    handed = eng.particles()
+    cpu = O.Oracle(scene, pots)
+    cpu.set_state(handed, eng.time)
+    assert cpu.step(5, 4) == 0
+    eng.to_device(1)
+    eng.step(5)
+    dev = eng.particles()
+    eng.from_device()
+    whole = O.Oracle(scene, pots)
+    assert whole.step(10, 4) == 0
+    pc, pw = cpu.particles(), whole.particles()
+    assert cpu.time == whole.time == eng.time == 10
+    for other in (dev, pw):
+        assert np.array_equal(pc["cid"], other["cid"])
+        assert np.abs(pc["x"] - other["x"]).max() <= 1e-9
+        assert np.abs(pc["v"] - other["v"]).max() <= 1e-7 * max(1.0, np.abs(other["v"]).max())
+        assert np.abs(pc["species"] - other["species"]).max() <= 1e-9

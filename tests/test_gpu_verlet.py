@@ -149,18 +149,6 @@ def test_list_mixed_matches_cell_mixed(tf, W, gpu):
     assert np.abs(pc["x"][m] - pl["x"][m]).max() <= 1e-4
 
 
-def test_list_capacity_overflow_is_an_error(tf, W, gpu):
-    scene = W.morse_lj(n=40_000)
-    scene["verlet_skin"] = 0.3
-    scene["verlet_capacity"] = 32
-    eng, _ = W.build_engine(scene)
-    eng.to_device(0)
-    assert eng.verlet_state()[2] == 32
-    with pytest.raises(tf.TfgError, match="capacity"):
-        eng.step(2)
-    eng.from_device()
-
-
 def test_list_fallbacks_and_ensemble(tf, W, O, gpu):
     """Systems the lists do not cover step with the cell pass; an ensemble
     of list systems matches each alone bitwise."""

@@ -94,6 +94,10 @@ _SIGNATURES = {
     "tfg_potential_harmonic": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 5),
     "tfg_potential_harmonic_angle": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 5),
     "tfg_potential_dpd": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 4 + [ctypes.c_int]),
+    "tfg_potential_overlapping_sphere": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 7),
+    "tfg_potential_glj": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_double] * 8 + [ctypes.c_int]),
+    "tfg_potential_from_table": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, _dp, ctypes.c_int, _dp, ctypes.c_int64]),
     "tfg_potential_destroy": (ctypes.c_int, [_vp]),
     "tfg_potential_info": (ctypes.c_int, [_vp, _ip, _u32p, _ip, _dp, _dp, _dp, _dp]),
     "tfg_potential_coeffs": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
@@ -116,6 +120,9 @@ _SIGNATURES = {
     "tfg_engine_from_device": (ctypes.c_int, [_vp]),
     "tfg_engine_on_device": (ctypes.c_int, [_vp]),
     "tfg_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "tfg_engine_set_blocks": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfg_engine_set_threads": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfg_engine_launch_config": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "tfg_engine_step": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tfg_nonbond_eval": (ctypes.c_int, [_vp]),
     "tfg_read_energies": (ctypes.c_int, [_vp, _dp]),
@@ -192,14 +199,13 @@ class EngineConfig:
     """tf.cuda.EngineConfig (source/cuda/tfEngineConfig.h:43-170, exported to
     Python by wraps/python/cuda.py) for one Engine: the device toggle and the
     launch settings under the reference's names.  set_blocks / set_threads
-    made while off the device apply at the next to_device, as in the
-    reference.  precision: PREC_MIXED (default) or PREC_FP64."""
+    apply at the next to_device and fail while the engine is on the device,
+    as in the reference (tfEngineConfig.cpp:98-124).  precision: PREC_MIXED
+    (default) or PREC_FP64."""
 
     def __init__(self, engine, precision=None):
         self._engine = engine
         self.precision = PREC_MIXED if precision is None else precision
-        self._blocks = 0
-        self._threads = 0
 
     def on_device(self):
         return self._engine.on_device
@@ -219,23 +225,25 @@ class EngineConfig:
 
     def to_device(self):
         self._engine.to_device(self.precision)
-        if self._blocks or self._threads:
-            self._engine.set_launch(self._blocks, self._threads)
 
     def from_device(self):
         self._engine.from_device()
 
     def set_blocks(self, numBlocks, deviceId=-1):
-        """Workgroups of the pair kernel (0: one cell per wave)."""
-        self._blocks = int(numBlocks)
-        if self.on_device():
-            self._engine.set_launch(self._blocks, 0)
+        """Workgroups of the pair kernel (0: one cell per wave), from the
+        next to_device; raises TfgError (E_STATE) while on the device."""
+        _check(lib().tfg_engine_set_blocks(self._engine._h, int(numBlocks)))
 
     def set_threads(self, numThreads, deviceId=-1):
-        """Threads per workgroup of the streaming kernels (a multiple of 64)."""
-        self._threads = int(numThreads)
-        if self.on_device():
-            self._engine.set_launch(0, self._threads)
+        """Threads per workgroup of the streaming kernels (a multiple of 64),
+        from the next to_device; raises TfgError (E_STATE) while on the device."""
+        _check(lib().tfg_engine_set_threads(self._engine._h, int(numThreads)))
+
+    def launch_config(self):
+        """(blocks, threads) that the next to_device applies."""
+        b, t = ctypes.c_int(), ctypes.c_int()
+        _check(lib().tfg_engine_launch_config(self._engine._h, ctypes.byref(b), ctypes.byref(t)))
+        return b.value, t.value
 
     def refresh_potentials(self):
         self._engine.refresh()
@@ -289,6 +297,28 @@ class Potential:
     @staticmethod
     def dpd(alpha=1.0, gamma=1.0, sigma=1.0, cutoff=1.0, shifted=False):
         return Potential._make(lib().tfg_potential_dpd, alpha, gamma, sigma, cutoff, 1 if shifted else 0)
+
+    @staticmethod
+    def overlapping_sphere(mu=1.0, kc=1.0, kh=0.0, r0=0.0, min=0.001, max=10.0, tol=0.001):
+        """Scaled potential of the distance r / (ri + rj) (tf.Potential.overlapping_sphere)."""
+        return Potential._make(lib().tfg_potential_overlapping_sphere, mu, kc, kh, r0, min, max, tol)
+
+    @staticmethod
+    def glj(e, m=3.0, n=None, k=0.0, r0=1.0, min=None, max=None, tol=0.01, shifted=False):
+        """Generalized Lennard-Jones (tf.Potential.glj): scaled unless shifted."""
+        n = 2.0 * m if n is None else n
+        mn = 0.05 * r0 if min is None else min
+        mx = 3.0 * r0 if max is None else max
+        return Potential._make(lib().tfg_potential_glj, e, m, n, k, r0, mn, mx, tol, 1 if shifted else 0)
+
+    @staticmethod
+    def from_table(flags, a, b, r0_plusone, alpha, n, c):
+        """A table potential from its own record (alpha[3], n, 8 (n + 1)
+        coefficients), evaluated exactly as given."""
+        al = _f64(alpha, 3)
+        cc = _f64(c).ravel()
+        return Potential._make(lib().tfg_potential_from_table, int(flags), float(a), float(b), float(r0_plusone),
+                               _dptr(al), int(n), _dptr(cc), cc.shape[0])
 
     def info(self):
         kind = ctypes.c_int()

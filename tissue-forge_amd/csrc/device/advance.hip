@@ -175,6 +175,25 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
     const int lane = threadIdx.x & 63;
     const int wstart = (blockIdx.x * blockDim.x + threadIdx.x) - lane;
     const int stride = gridDim.x * blockDim.x;
+    // Halted (the update writes its own failure to err[1], not to this word).
+    const int halt = P.err[0];
+    if (halt & kHaltTooFast) return;
+    if (halt != 0) {
+        // A list overflow in this step: no update.  The binning of this step
+        // reordered the state into A and the id map with it, so buffer B
+        // takes A as it is, and the state stays that of the last good step.
+        for (int i = wstart + lane; i < P.np; i += stride) {
+            P.B.pos[i] = P.A.pos[i];
+            P.B.vel[i] = P.A.vel[i];
+            P.B.meta[i] = P.A.meta[i];
+            P.B.imass[i] = P.A.imass[i];
+            for (int k = 0; k < S; k++) {
+                P.B.sp[(size_t)k * P.np + i] = P.A.sp[(size_t)k * P.np + i];
+                P.B.q[(size_t)k * P.np + i] = P.A.q[(size_t)k * P.np + i];
+            }
+        }
+        return;
+    }
     for (int base = wstart; base < P.np; base += stride) {
         const int i = base + lane;
         const bool valid = i < P.np;
@@ -324,7 +343,7 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                         moved = true;
                         for (int k = 0; k < 3; k++) {
                             if (xv[k] >= real(2.0 * sd.h[k] - sd.foff[k]) || xv[k] <= real(-sd.h[k] - sd.foff[k]))
-                                atomicOr(P.err, 1);
+                                atomicOr(P.err + 1, 1);
                             xv[k] = real(static_cast<double>(xv[k]) - delta[k] * sd.h[k]);
                             loc[k] = (loc[k] + delta[k] + sd.cdim[k]) % sd.cdim[k];
                         }
@@ -407,7 +426,7 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                     delta[k] = (xv[k] >= real(sd.h[k] - sd.foff[k])) - (xv[k] < real(-sd.foff[k]));
                     far = far || xv[k] >= real(2.0 * sd.h[k] - sd.foff[k]) || xv[k] <= real(-sd.h[k] - sd.foff[k]);
                 }
-                if (far) atomicOr(P.err, 1);
+                if (far) atomicOr(P.err + 1, 1);
                 if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
                     moved = true;
                     for (int k = 0; k < 3; k++) {
@@ -500,7 +519,9 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
 // angle, then up to 3 species totals... species use their own kernel.
 __global__ __launch_bounds__(256) void k_energy_part(const SysDesc *sys, const double *e_nb, const double *e_vir,
                                                      const double *e_kin, const double *e_bond,
-                                                     const double *e_ang, double *part, int nbx) {
+                                                     const double *e_ang, double *part, int nbx,
+                                                     const int *halt) {
+    if (*halt != 0) return;
     __shared__ double s[5][256];
     const SysDesc &sd = sys[blockIdx.y];
     const int b0 = blockIdx.x * kEnergyTile;
@@ -526,7 +547,8 @@ __global__ __launch_bounds__(256) void k_energy_part(const SysDesc *sys, const d
         for (int k = 0; k < 5; k++) part[((size_t)blockIdx.y * nbx + blockIdx.x) * 5 + k] = s[k][0];
 }
 
-__global__ __launch_bounds__(256) void k_energy_final(const double *part, int nbx, double *out) {
+__global__ __launch_bounds__(256) void k_energy_final(const double *part, int nbx, double *out, const int *halt) {
+    if (*halt != 0) return;
     __shared__ double s[5][256];
     double a[5] = {0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < nbx; b += 256)
@@ -544,7 +566,9 @@ __global__ __launch_bounds__(256) void k_energy_final(const double *part, int nb
 
 // Species totals per system in FP64 (conservation check of the flux pass).
 template <typename real>
-__global__ __launch_bounds__(256) void k_species_total(const SysDesc *sys, const real *sp, int np, int S, double *out) {
+__global__ __launch_bounds__(256) void k_species_total(const SysDesc *sys, const real *sp, int np, int S, double *out,
+                                                       const int *halt) {
+    if (*halt != 0) return;
     __shared__ double s[256];
     const SysDesc &sd = sys[blockIdx.x];
     for (int k = 0; k < S; k++) {
@@ -564,6 +588,7 @@ __global__ __launch_bounds__(256) void k_species_total(const SysDesc *sys, const
 // Fluxes_integrate over every particle (tfFlux.cpp:175-202), in place on
 // buffer A: the flux sub-steps of tf_engine_advance.cpp:313-323.
 template <typename real> __global__ __launch_bounds__(256) void k_flux_integrate(KP<real> P) {
+    if (P.err[0] != 0) return;
     const size_t n = (size_t)P.np * P.S;
     for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (size_t)gridDim.x * blockDim.x) {
         const int i = static_cast<int>(o % P.np);
@@ -593,11 +618,11 @@ template <typename real> int run_advance(DeviceEnsemble *E, const KP<real> &P, i
 template <typename real> int run_energy(DeviceEnsemble *E, const KP<real> &P) {
     const int nbx = E->max_sys_blocks;
     hipLaunchKernelGGL(k_energy_part, dim3(nbx, E->nsys), dim3(256), 0, E->stream, E->d_sys, E->e_nb, E->e_vir,
-                       E->e_kin, E->e_bond, E->e_ang, E->e_part, nbx);
-    hipLaunchKernelGGL(k_energy_final, dim3(E->nsys), dim3(256), 0, E->stream, E->e_part, nbx, E->d_energy);
+                       E->e_kin, E->e_bond, E->e_ang, E->e_part, nbx, E->d_err);
+    hipLaunchKernelGGL(k_energy_final, dim3(E->nsys), dim3(256), 0, E->stream, E->e_part, nbx, E->d_energy, E->d_err);
     if (E->S > 0)
         hipLaunchKernelGGL(k_species_total<real>, dim3(E->nsys), dim3(256), 0, E->stream, E->d_sys, P.B.sp, E->np,
-                           E->S, E->d_spec_tot);
+                           E->S, E->d_spec_tot, E->d_err);
     return launch_error("energy");
 }
 

@@ -25,6 +25,10 @@ constexpr int kScanBlock = 1024;   // threads of a scan block
 constexpr int kScanPer = 4;        // counts per thread
 constexpr int kScanTile = kScanBlock * kScanPer;
 
+// A binning kernel does nothing when the step keeps its order (gate is the
+// rebuild flag in list mode, null otherwise) or the ensemble is halted.
+__device__ inline bool bin_skip(const int *gate, const int *halt) { return (gate && *gate == 0) || *halt != 0; }
+
 __device__ inline int block_exclusive_scan(int v, int *s) {
     // 1024 threads: wave-level inclusive scan with shuffles, then wave totals.
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -49,8 +53,9 @@ __device__ inline int block_exclusive_scan(int v, int *s) {
     return before + x - v;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int *cnt, int *bsum, int n, const int *gate) {
-    if (gate && *gate == 0) return;
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int *cnt, int *bsum, int n, const int *gate,
+                                                           const int *halt) {
+    if (bin_skip(gate, halt)) return;
     __shared__ int s[16];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
     int v = 0;
@@ -61,8 +66,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int *cnt, int 
 }
 
 // Exclusive scan of the tile totals in place; bsum[nb] = grand total.
-__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(int *bsum, int nb, const int *gate) {
-    if (gate && *gate == 0) return;
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(int *bsum, int nb, const int *gate, const int *halt) {
+    if (bin_skip(gate, halt)) return;
     __shared__ int s[16];
     int carry = 0;
     for (int b0 = 0; b0 < nb; b0 += kScanBlock) {
@@ -80,8 +85,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_tiles(int *bsum, int nb, co
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int *cnt, const int *bsum, int *start, int n, int nb,
-                                                          const int *gate) {
-    if (gate && *gate == 0) return;
+                                                          const int *gate, const int *halt) {
+    if (bin_skip(gate, halt)) return;
     __shared__ int s[16];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
     int c[kScanPer];
@@ -99,8 +104,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int *cnt, const
 }
 
 // Counts per cell from buffer B (used once after an upload).
-__global__ void k_count(const int4 *meta, int *cnt, int np, const int *gate) {
-    if (gate && *gate == 0) return;
+__global__ void k_count(const int4 *meta, int *cnt, int np, const int *gate, const int *halt) {
+    if (bin_skip(gate, halt)) return;
     const int lane = threadIdx.x & 63;
     const int wstart = (blockIdx.x * blockDim.x + threadIdx.x) - lane;
     const int stride = gridDim.x * blockDim.x;
@@ -116,8 +121,9 @@ __global__ void k_count(const int4 *meta, int *cnt, int np, const int *gate) {
 
 // Places particle i of buffer B at start[cell] + (reserved offset); one
 // atomic per run of equal cell ids in the wave.
-__global__ void k_scatter(const int4 *meta, const int *start, int *fill, int2 *tmp, int np, const int *gate) {
-    if (gate && *gate == 0) return;
+__global__ void k_scatter(const int4 *meta, const int *start, int *fill, int2 *tmp, int np, const int *gate,
+                          const int *halt) {
+    if (bin_skip(gate, halt)) return;
     const int lane = threadIdx.x & 63;
     const int wstart = (blockIdx.x * blockDim.x + threadIdx.x) - lane;
     const int stride = gridDim.x * blockDim.x;
@@ -141,8 +147,8 @@ __global__ void k_scatter(const int4 *meta, const int *start, int *fill, int2 *t
 // 16-lane groups with shuffles, four cells per wave; larger cells use the
 // whole wave.
 __global__ __launch_bounds__(64) void k_cellorder(const int *start, const int2 *tmp, int *perm, int *cnt,
-                                                  int *fill, int ncells, const int *gate) {
-    if (gate && *gate == 0) return;
+                                                  int *fill, int ncells, const int *gate, const int *halt) {
+    if (bin_skip(gate, halt)) return;
     __shared__ int keys[2048];
     const int lane = threadIdx.x;
     const int g = lane >> 4, e = lane & 15;
@@ -192,6 +198,7 @@ __global__ void k_permute(KP<real> P) {
     using v4 = typename V<real>::v4;
     constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
     const int S = P.S;
+    if (P.err[0] != 0) return;   // halted
     const bool any = !P.list_mode || P.lflags[0] != 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.np; i += gridDim.x * blockDim.x) {
         // slot i belongs to the system of the particle buffer B holds there;
@@ -229,7 +236,7 @@ __global__ void k_permute(KP<real> P) {
 
 int run_count(DeviceEnsemble *E) {
     hipLaunchKernelGGL(k_count, dim3(E->stream_grid()), dim3(E->shape.stream_threads), 0, E->stream, E->meta[1],
-                       E->cnt, E->np, nullptr);
+                       E->cnt, E->np, nullptr, E->d_err);
     return launch_error("count");
 }
 
@@ -239,14 +246,14 @@ template <typename real> int run_bin(DeviceEnsemble *E, const KP<real> &P) {
     const int g = E->stream_grid(), t = E->shape.stream_threads;
     const int *gate = E->list_mode ? E->d_lflags : nullptr;
     if (E->list_mode)
-        hipLaunchKernelGGL(k_count, dim3(g), dim3(t), 0, s, E->meta[1], E->cnt, E->np, gate);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->ncells, gate);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanBlock), 0, s, E->bsum, nb, gate);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->start, E->ncells, nb, gate);
-    hipLaunchKernelGGL(k_scatter, dim3(g), dim3(t), 0, s, E->meta[1], E->start, E->fill, E->tmp, E->np, gate);
+        hipLaunchKernelGGL(k_count, dim3(g), dim3(t), 0, s, E->meta[1], E->cnt, E->np, gate, E->d_err);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->ncells, gate, E->d_err);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanBlock), 0, s, E->bsum, nb, gate, E->d_err);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->start, E->ncells, nb, gate, E->d_err);
+    hipLaunchKernelGGL(k_scatter, dim3(g), dim3(t), 0, s, E->meta[1], E->start, E->fill, E->tmp, E->np, gate, E->d_err);
     const int oc = std::min((E->ncells + 3) / 4, 65536);
     hipLaunchKernelGGL(k_cellorder, dim3(oc), dim3(64), 0, s, E->start, E->tmp, E->perm, E->cnt, E->fill, E->ncells,
-                       gate);
+                       gate, E->d_err);
     hipLaunchKernelGGL(k_permute<real>, dim3(g), dim3(t), 0, s, P);
     return launch_error("binning");
 }

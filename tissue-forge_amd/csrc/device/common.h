@@ -64,6 +64,10 @@ struct SysDesc {
     double skin_half2;   // (skin / 2)^2, narrowed by a relative 1e-4
 };
 
+// Halt bits of the device error word (KP::err[0]).
+constexpr int kHaltTooFast = 1;    // a particle moved two cells or more (space_shuffle_local's error)
+constexpr int kHaltOverflow = 2;   // a neighbour list exceeded its capacity
+
 // Neighbour-list entry: the partner's slot, and its build cell's offset from
 // the particle's build cell as ((ox + 2) * 5 + (oy + 2)) * 5 + (oz + 2).
 constexpr int kListSlotBits = 25;
@@ -148,6 +152,11 @@ template <typename real> struct KP {
     const int *inc_off, *inc;
     const int4 *bonds, *angles;
     long long *step;
+    // err[0]: halt word, read by every kernel of a step at its start; bit
+    //   kHaltTooFast (folded in from err[1] by the next step's tick) or
+    //   kHaltOverflow (set by the list build).  A step that finds it set
+    //   does nothing, so the state stays where the failing step left it.
+    // err[1]: set by the update when a particle moved two cells or more.
     int *err;
     int ncells, np, S, energy;
     int step_ahead;   // 1 when forces are evaluated outside a step (time + 1)

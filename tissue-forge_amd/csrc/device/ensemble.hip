@@ -85,20 +85,29 @@ template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy) {
     return P;
 }
 
-__global__ void k_tick(long long *step) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+// Start of a step: a failure of the last update (err[1]) becomes a halt;
+// a halted ensemble does not count the step.
+__device__ inline bool tick_halted(int *err) {
+    if (err[1] != 0) err[0] |= kHaltTooFast;
+    return err[0] != 0;
+}
+
+__global__ void k_tick(long long *step, int *err) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !tick_halted(err)) *step += 1;
 }
 
 // List mode: a system rebuilds in this step when the last advance asked for
 // it (or its state was just uploaded or rebinned); the requests are
 // consumed.  lflags[0] tells the binning kernels that some system rebuilds.
-__global__ __launch_bounds__(256) void k_tick_list(long long *step, int *lflags, int *sflags, int nsys) {
-    __shared__ int any;
+__global__ __launch_bounds__(256) void k_tick_list(long long *step, int *lflags, int *sflags, int nsys, int *err) {
+    __shared__ int any, halted;
     if (threadIdx.x == 0) {
-        *step += 1;
+        halted = tick_halted(err) ? 1 : 0;
+        if (!halted) *step += 1;
         any = 0;
     }
     __syncthreads();
+    if (halted) return;
     int mine = 0;
     for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
         const int rb = sflags[s] != 0 ? 1 : 0;
@@ -137,7 +146,8 @@ template <typename real> static int enqueue_step(DeviceEnsemble *E, bool energy,
     if (E->list_mode) {
         // Neighbour lists: bin (or copy) B into A, rebuild the lists when the
         // flag is up, list pairs, advance (which raises the flag).
-        hipLaunchKernelGGL(k_tick_list, dim3(1), dim3(256), 0, E->stream, E->d_step, E->d_lflags, E->d_sflags, E->nsys);
+        hipLaunchKernelGGL(k_tick_list, dim3(1), dim3(256), 0, E->stream, E->d_step, E->d_lflags, E->d_sflags, E->nsys,
+                           E->d_err);
         if (ev) TFG_HIP(hipEventRecord(ev[0], E->stream));
         if ((rc = run_bin<real>(E, Pe)) != TFG_OK) return rc;
         if (ev) TFG_HIP(hipEventRecord(ev[1], E->stream));
@@ -150,7 +160,7 @@ template <typename real> static int enqueue_step(DeviceEnsemble *E, bool energy,
         if (energy && (rc = run_energy<real>(E, Pe)) != TFG_OK) return rc;
         return TFG_OK;
     }
-    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, E->stream, E->d_step);
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, E->stream, E->d_step, E->d_err);
     const bool rk4 = E->integrator == INTEG_RUNGE_KUTTA_4;
     const int rounds = rk4 ? 4 : 1;
     for (int st = 0; st < rounds; st++) {
@@ -331,7 +341,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
             const double half = 0.5 * e.verlet_skin * (1.0 - 1e-4);
             d.skin_half2 = half * half;
             double vol = 1.0;
-            for (int k = 0; k < 3; k++) vol *= e.geom.dim[k] - e.geom.origin[k];
+            for (int k = 0; k < 3; k++) vol *= e.geom.cdim[k] * e.geom.h[k];   // the volume the cell grid spans
             const double expect = (vol > 0 ? e.parts.size() / vol : 0.0) * 4.0 / 3.0 * M_PI * rcs * rcs * rcs;
             const int cap = e.verlet_capacity > 0 ? e.verlet_capacity : static_cast<int>(2.0 * expect) + 64;
             lcap = std::max(lcap, (cap + 31) / 32 * 32);
@@ -571,7 +581,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     ALLOC(E->d_energy, double, 5 * nsys);
     ALLOC(E->d_spec_tot, double, kMaxSpecies * nsys);
     ALLOC(E->d_step, long long, 1);
-    ALLOC(E->d_err, int, 1);
+    ALLOC(E->d_err, int, 2);
     ALLOC(E->inc_off, int, np + 1);
     ALLOC(E->inc, int, inc.size());
     ALLOC(E->d_bonds, int4, hb.size());
@@ -624,7 +634,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     }
     // Zeroed state (outside any graph capture).
     TFG_HIP(hipMemset(E->d_step, 0, sizeof(long long)));
-    TFG_HIP(hipMemset(E->d_err, 0, sizeof(int)));
+    TFG_HIP(hipMemset(E->d_err, 0, 2 * sizeof(int)));
     TFG_HIP(hipMemset(E->cnt, 0, sizeof(int) * nc));
     TFG_HIP(hipMemset(E->fill, 0, sizeof(int) * nc));
     TFG_HIP(hipMemset(E->force, 0, sizeof(v4) * (size_t)np));
@@ -771,24 +781,34 @@ template <typename real> static int step_t(DeviceEnsemble *E, int nsteps, bool u
             E->prof_steps = nsteps;
         }
     }
-    int err = 0;
-    TFG_HIP(hipMemcpy(&err, E->d_err, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) {
-        TFG_HIP(hipMemset(E->d_err, 0, sizeof(int)));
-        return set_error(TFG_E_TOOFAST, "particle moving too fast (moved two cells or more in one step)");
-    }
-    if (E->list_mode) {
-        int fl[4];
-        TFG_HIP(hipMemcpy(fl, E->d_lflags, sizeof(fl), hipMemcpyDeviceToHost));
-        if (fl[2]) {
-            fl[2] = 0;
-            TFG_HIP(hipMemcpy(E->d_lflags, fl, sizeof(fl), hipMemcpyHostToDevice));
-            std::vector<int> req(E->nsys, 1);   // every system rebuilds at the next step
+    // A failed step halts the ensemble (every later kernel returns at
+    // once), so the state is the one the failing step left: after the update
+    // for a particle moving too fast, before the pair pass for a list
+    // overflow, as the reference's engine_step returns after time += 1
+    // (tfEngine.cpp:1388-1402, tf_engine_advance.cpp:482-489).
+    int err[2] = {0, 0};
+    TFG_HIP(hipMemcpy(err, E->d_err, sizeof(err), hipMemcpyDeviceToHost));
+    if (err[1]) err[0] |= kHaltTooFast;
+    if (err[0]) {
+        TFG_HIP(hipMemset(E->d_err, 0, sizeof(err)));
+        if (E->list_mode) {   // every system rebuilds at the next step
+            std::vector<int> req(E->nsys, 1);
             TFG_HIP(hipMemcpy(E->d_sflags, req.data(), sizeof(int) * E->nsys, hipMemcpyHostToDevice));
-            return set_error(TFG_E_RANGE, "neighbour list capacity (" + std::to_string(E->lcap) +
-                                              " entries per particle) exceeded; raise it with tfg_engine_set_verlet");
         }
+        if (err[0] & kHaltTooFast)
+            return set_error(TFG_E_TOOFAST, "particle moving too fast (moved two cells or more in one step)");
+        return set_error(TFG_E_RANGE, "neighbour list capacity (" + std::to_string(E->lcap) +
+                                          " entries per particle) exceeded; raise it with tfg_engine_set_verlet");
     }
+    return TFG_OK;
+}
+
+// Step counter of every system from the device (engine->time): the steps
+// begun since the upload, a failing one included.
+static int sync_time(DeviceEnsemble *E) {
+    long long steps = 0;
+    TFG_HIP(hipMemcpy(&steps, E->d_step, sizeof(steps), hipMemcpyDeviceToHost));
+    for (int s = 0; s < E->nsys; s++) E->engines[s]->time = E->sys[s].time0 + steps;
     return TFG_OK;
 }
 
@@ -829,10 +849,13 @@ int ensemble_step(DeviceEnsemble *E, int nsteps, bool use_graph) {
     if (nsteps < 0) return set_error(TFG_E_INVALIDARG, "negative step count");
     if (nsteps == 0) return TFG_OK;
     TFG_HIP(hipSetDevice(E->device));
-    int rc = E->precision == PREC_MIXED ? step_t<float>(E, nsteps, use_graph) : step_t<double>(E, nsteps, use_graph);
-    if (rc != TFG_OK) return rc;
-    for (Engine *e : E->engines) e->time += nsteps;
-    return TFG_OK;
+    const int rc = E->precision == PREC_MIXED ? step_t<float>(E, nsteps, use_graph) : step_t<double>(E, nsteps, use_graph);
+    if (rc == TFG_E_DEVICE) return rc;
+    // also after a failed step: the host time counts the steps begun
+    const std::string msg = rc != TFG_OK ? std::string(last_error()) : std::string();
+    const int rt = sync_time(E);
+    if (rt != TFG_OK) return rt;
+    return rc != TFG_OK ? set_error(rc, msg) : TFG_OK;
 }
 
 // Force evaluation without the update: binning, nonbonded and bonded passes;

@@ -201,6 +201,13 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
     if (BUILD && P.lflags[0] == 0) return;
     constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
     extern __shared__ __align__(16) unsigned char s_dyn[];
+    // A halted ensemble does nothing.  The list build raises the halt itself
+    // (overflow), so the word is read once per workgroup and shared: every
+    // wave of the workgroup takes the same branch before its barriers.
+    __shared__ int s_halt;
+    if (threadIdx.x == 0) s_halt = __atomic_load_n(P.err, __ATOMIC_RELAXED);
+    __syncthreads();
+    if (s_halt != 0) return;
     __shared__ FluxDesc<real> s_flux[FLUX ? kMaxFluxEntries : 1];
     __shared__ int2 s_fmap[FLUX ? kMaxTypes * kMaxTypes : 1];
 
@@ -802,7 +809,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                 if constexpr (BUILD) {
                     if (active && sl == 0) {
                         P.lcount[si] = lbase;
-                        if (lbase > P.lcap) atomicOr(&P.lflags[2], 1);
+                        if (lbase > P.lcap) atomicOr(P.err, kHaltOverflow);
                     }
                     continue;
                 }
@@ -1000,6 +1007,7 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     // Workgroups go to the 8 XCDs in turn: the workgroups of one XCD take
     // consecutive particle blocks (the grid is a multiple of 8), so that the
     // partner positions they gather meet in that XCD's L2.
+    if (P.err[0] != 0) return;   // halted (nothing writes the word during this kernel)
     const int nblk = gridDim.x;
     const int b = (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8;
     const int4 lb = P.lblock[b];

@@ -370,6 +370,60 @@ int tfg_potential_dpd(tfg_potential **out, double alpha, double gamma, double si
     return TFG_OK;
 }
 
+int tfg_potential_overlapping_sphere(tfg_potential **out, double mu, double kc, double kh, double r0, double min,
+                                     double max, double tol) {
+    if (!out) return set_error(TFG_E_INVALIDARG, "null output");
+    if (!(max > min) || !(min > 0)) return set_error(TFG_E_INVALIDARG, "need 0 < min < max");
+    TFG_GUARD({
+        tfg_potential *p = new tfg_potential();
+        const double t = tol < 0 ? 0.001 : tol;
+        return finish_potential(out, p, create_overlapping_sphere(p->p, mu, kc, kh, r0, min, max, t));
+    });
+}
+
+int tfg_potential_glj(tfg_potential **out, double e, double m, double n, double k, double r0, double min, double max,
+                      double tol, int shifted) {
+    if (!out) return set_error(TFG_E_INVALIDARG, "null output");
+    if (!(max > min) || !(min > 0)) return set_error(TFG_E_INVALIDARG, "need 0 < min < max");
+    if (!(n != m)) return set_error(TFG_E_INVALIDARG, "the exponents m and n must differ");
+    TFG_GUARD({
+        tfg_potential *p = new tfg_potential();
+        const double t = tol < 0 ? 0.01 : tol;
+        return finish_potential(out, p, create_glj(p->p, e, m, n, k, r0, min, max, t, shifted != 0));
+    });
+}
+
+int tfg_potential_from_table(tfg_potential **out, uint32_t flags, double a, double b, double r0_plusone,
+                             const double alpha[3], int n, const double *c, int64_t ncoef) {
+    if (!out || !alpha || (ncoef > 0 && !c)) return set_error(TFG_E_INVALIDARG, "null argument");
+    constexpr uint32_t kKnown = 0xFFFFu;   // PotentialFlags of tfPotential.h:56-108 (bits 0-15)
+    if (flags & ~kKnown) return set_error(TFG_E_INVALIDARG, "unknown potential flag bits");
+    if (n < 1 || n > kIntervalsMax) return set_error(TFG_E_INVALIDARG, "table interval count must be in [1, 3048]");
+    if (ncoef != static_cast<int64_t>(kTableChunk) * (n + 1))
+        return set_error(TFG_E_INVALIDARG, "a table of n intervals has 8 (n + 1) coefficients");
+    TFG_GUARD({
+        tfg_potential *p = new tfg_potential();
+        Potential &q = p->p;
+        q.kind = KIND_TABLE;
+        q.flags = flags;
+        q.a = a;
+        q.b = b;
+        q.r0_plusone = r0_plusone;
+        for (int k = 0; k < 3; k++) q.alpha[k] = alpha[k];
+        q.alpha[3] = 0.0;
+        q.n = n;
+        q.c.assign(c, c + ncoef);
+        q.name = "table";
+        const std::string why = check_potential(q);
+        if (!why.empty()) {
+            delete p;
+            return set_error(TFG_E_INVALIDARG, why);
+        }
+        *out = p;
+    });
+    return TFG_OK;
+}
+
 int tfg_potential_destroy(tfg_potential *p) {
     delete p;
     return TFG_OK;
@@ -462,7 +516,14 @@ int tfg_engine_to_device(tfg_engine *h, int precision) {
     if (h->e.device) return TFG_OK;   // tfEngine.cpp:1148-1151: already on the device
     Engine *ep = &h->e;
     DeviceEnsemble *d = nullptr;
-    TFG_GUARD(return ensemble_create(&d, &ep, 1, precision));
+    TFG_GUARD({
+        const int rc = ensemble_create(&d, &ep, 1, precision);
+        if (rc != TFG_OK) return rc;
+        // the launch settings made off the device (setBlocks / setThreads)
+        if (h->e.launch_blocks > 0 || h->e.launch_threads > 0)
+            return ensemble_set_launch(d, h->e.launch_blocks, h->e.launch_threads);
+    });
+    return TFG_OK;
 }
 
 int tfg_engine_from_device(tfg_engine *h) {
@@ -474,6 +535,30 @@ int tfg_engine_from_device(tfg_engine *h) {
 }
 
 int tfg_engine_on_device(const tfg_engine *h) { return h && h->e.device ? 1 : 0; }
+
+int tfg_engine_set_blocks(tfg_engine *h, int blocks) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");   // TFCUDAERR_ondevice
+    if (blocks < 0) return set_error(TFG_E_INVALIDARG, "negative workgroup count");
+    h->e.launch_blocks = blocks;
+    return TFG_OK;
+}
+
+int tfg_engine_set_threads(tfg_engine *h, int threads) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device");   // TFCUDAERR_ondevice
+    if (threads < 0 || threads % 64 != 0 || threads > 1024)
+        return set_error(TFG_E_INVALIDARG, "threads must be a multiple of 64 up to 1024");
+    h->e.launch_threads = threads;
+    return TFG_OK;
+}
+
+int tfg_engine_launch_config(const tfg_engine *h, int *blocks, int *threads) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    if (blocks) *blocks = h->e.launch_blocks;
+    if (threads) *threads = h->e.launch_threads;
+    return TFG_OK;
+}
 
 int tfg_set_launch(tfg_engine *h, int blocks, int threads) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");

@@ -584,6 +584,9 @@ int engine_from_json(Engine &e, const char *text, size_t len) {
             pt.dpd_sigma = dpd[2];
             const auto &c = nums_of(pj, "c", pt.kind == KIND_TABLE ? static_cast<long long>(kTableChunk) * (n + 1) : -1);
             pt.c = c;
+            // what the device evaluation relies on (interval range, finite map and coefficients)
+            const std::string why = check_potential(pt);
+            if (!why.empty()) throw ParseError{"potential \"" + pt.name + "\": " + why};
             // kept as saved (no de-duplication), so the saved ids stay valid
             e.pots.push_back(std::make_shared<Potential>(std::move(pt)));
         }

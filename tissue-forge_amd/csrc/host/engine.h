@@ -141,6 +141,9 @@ class Engine {
     double verlet_skin = 0.0;
     int verlet_capacity = 0;
     std::vector<int> wallpot;      // 6 * max_types: pot id of (face, type) or -1
+    // Launch settings applied at the next upload (cuda::EngineConfig::
+    // setBlocks / setThreads, tfEngineConfig.cpp:98-124); 0 keeps the default.
+    int launch_blocks = 0, launch_threads = 0;
     int64_t time = 0;                  // step counter (engine->time)
     int dpd_noise = DPD_NOISE_UNIT;
     uint64_t dpd_seed = 0;

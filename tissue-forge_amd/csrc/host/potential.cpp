@@ -440,6 +440,97 @@ void create_dpd(Potential &p, double alpha, double gamma, double sigma, double c
     if (shifted) p.flags |= POT_SHIFTED;
 }
 
+int create_overlapping_sphere(Potential &p, double mu, double k, double harmonic_k, double harmonic_r0,
+                              double min, double max, double tol) {
+    p = Potential();
+    p.flags = POT_R2 | 1u;   // POTENTIAL_R2 | POTENTIAL_LJ126
+    p.name = harmonic_k == 0.0 ? "Overlapping Sphere" : "Overlapping Sphere with Harmonic";
+    p.r0_plusone = harmonic_r0 + 1;
+    Fn f = [=](double x) {
+        const double core = x <= 1 ? -x + x * std::log(x)
+                                   : -1 + std::pow(k, -2) - (std::pow(M_E, k - k * x) * (1 + k * (-1 + x))) / std::pow(k, 2);
+        return mu * core + harmonic_k * std::pow(-x + harmonic_r0, 2);
+    };
+    Fn fp = [=](double x) {
+        const double core = x <= 1 ? std::log(x) : std::pow(M_E, k * (1 - x)) * (-1 + x);
+        return mu * core + 2 * harmonic_k * (-x + harmonic_r0);
+    };
+    Fn f6 = [=](double x) {
+        const double core = x <= 1 ? 24 / std::pow(x, 5)
+                                   : std::pow(M_E, k - k * x) * std::pow(k, 4) -
+                                         std::pow(M_E, k - k * x) * std::pow(k, 4) * (-4 - k + k * x);
+        return mu * core;
+    };
+    const int rc = build_table(p, f, fp, f6, min, max, tol);
+    p.flags |= POT_SCALED;
+    return rc;
+}
+
+int create_glj(Potential &p, double e, double m, double n, double k, double r0, double min, double max, double tol,
+               bool shifted) {
+    p = Potential();
+    p.flags = POT_R2 | 1u;
+    p.name = "Generalized Lennard-Jones";
+    p.r0_plusone = r0 + 1;
+    Fn f = [=](double r) {
+        return k * std::pow(-r + r0, 2) + (e * (-(n * std::pow(r0 / r, m)) + m * std::pow(r0 / r, n))) / (-m + n);
+    };
+    Fn fp = [=](double r) {
+        return 2 * k * (-r + r0) +
+               (e * ((m * n * r0 * std::pow(r0 / r, -1 + m)) / std::pow(r, 2) -
+                     (m * n * r0 * std::pow(r0 / r, -1 + n)) / std::pow(r, 2))) /
+                   (-m + n);
+    };
+    // Sixth derivative of the two power terms (the harmonic part has none),
+    // in the reference's expanded form (tfPotential.cpp:2122-2138): for
+    // u = r0 / r, d^6 u^q / dr^6 = sum_j w_j [q]_j r0^j u^(q-j) / r^(6+j) with
+    // the falling factorial [q]_j = q (q-1) ... (q-j+1) and weights
+    // w_6..w_1 = 1, 30, 300, 1200, 1800, 720.
+    Fn f6 = [=](double r) {
+        auto d6 = [&](double q) {
+            static const double w[7] = {0, 720, 1800, 1200, 300, 30, 1};
+            double s = 0;
+            for (int j = 6; j >= 1; j--) {
+                double fall = 1;
+                for (int t = 0; t < j; t++) fall *= q - t;
+                s += w[j] * fall * std::pow(r0, j) * std::pow(r0 / r, q - j) / std::pow(r, 6 + j);
+            }
+            return s;
+        };
+        return e * (-(n * d6(m)) + m * d6(n)) / (-m + n);
+    };
+    const int rc = build_table(p, f, fp, f6, min, max, tol);
+    if (shifted) {
+        p.flags |= POT_SHIFTED;
+        p.r0_plusone = r0 + 1;
+    } else {
+        p.flags |= POT_SCALED;
+    }
+    return rc;
+}
+
+std::string check_potential(const Potential &p) {
+    auto fin = [](double v) { return std::isfinite(v); };
+    if (p.kind == KIND_DPD) {
+        if (!fin(p.a) || !fin(p.b) || !(p.b > 0)) return "DPD range must be finite with a positive cutoff";
+        if (!fin(p.dpd_alpha) || !fin(p.dpd_gamma) || !fin(p.dpd_sigma)) return "DPD coefficients must be finite";
+        return "";
+    }
+    if (p.kind != KIND_TABLE) return "unknown potential kind";
+    if (p.n < 1 || p.n > kIntervalsMax) return "table interval count must be in [1, 3048]";
+    if (!fin(p.a) || !fin(p.b) || !(p.a < p.b)) return "table range must be finite with a < b";
+    if (!fin(p.r0_plusone)) return "r0 + 1 must be finite";
+    for (int k = 0; k < 3; k++)
+        if (!fin(p.alpha[k])) return "table interval map must be finite";
+    if (p.c.size() != static_cast<size_t>(kTableChunk) * (p.n + 1)) return "table needs 8 (n + 1) coefficients";
+    // intervals 1..n; interval 0 continues the table left of a and is not
+    // finite for a table that starts at r = 0 (the evaluation never reaches it
+    // for r >= a)
+    for (size_t k = kTableChunk; k < p.c.size(); k++)
+        if (!fin(p.c[k])) return "table coefficients must be finite";
+    return "";
+}
+
 static inline void horner(const double *c, double x, double *e, double *eff) {
     double ee = c[2] * x + c[3];
     double ef = c[2];

@@ -75,6 +75,22 @@ int create_harmonic_angle(Potential &p, double k, double theta0, double min,
                           double max, double tol);
 void create_dpd(Potential &p, double alpha, double gamma, double sigma,
                 double cutoff, bool shifted);
+// Potential::overlapping_sphere (tfPotential.cpp:1224-1307, 3086-3102):
+// mu (x ln x - x) below x = 1, the exponential tail above, plus
+// harmonic_k (x - harmonic_r0)^2, in the scaled distance x = r / (ri + rj).
+int create_overlapping_sphere(Potential &p, double mu, double k, double harmonic_k,
+                              double harmonic_r0, double min, double max, double tol);
+// Potential::glj (tfPotential.cpp:2085-2185, 3047-3066): generalized
+// Lennard-Jones e (m (r0/r)^n - n (r0/r)^m) / (n - m) + k (r - r0)^2,
+// shifted (r - (ri + rj) + r0 + 1) or scaled (r / (ri + rj)).
+int create_glj(Potential &p, double e, double m, double n, double k, double r0,
+               double min, double max, double tol, bool shifted);
+
+// Checks a table or DPD record for what the device evaluation relies on:
+// for a table 1 <= n <= kIntervalsMax, 8 (n + 1) finite coefficients, a
+// finite range a < b and a finite interval map.  Returns "" when valid,
+// else what is wrong.
+std::string check_potential(const Potential &p);
 
 // FP64 evaluation of a table at squared distance r2 with particle radii ri,
 // rj (potential_eval_ex semantics).  Returns false when out of range.

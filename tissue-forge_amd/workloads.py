@@ -28,6 +28,12 @@ Workloads (particle counts are the full sizes; every builder takes a scale):
                   potential + free-slip wall, Morse walls for both types and
                   a harmonic top wall for type B (after
                   examples/py/boundary_morse.py).
+  radius_mix      three types of radii 0.3, 0.45, 0.6 whose potentials map
+                  the pair distance through the particle radii: Morse with
+                  the reference's defaults (shifted: evaluated at
+                  r - (ri + rj) + r0 + 1), the generalized Lennard-Jones with
+                  its defaults (scaled: r / (ri + rj)), an overlapping-sphere
+                  potential (scaled) and a shifted DPD potential.
 """
 import math
 
@@ -155,6 +161,40 @@ def walled_morse(n=8000, L=20.0, cells=4, seed=7, dt=0.005, dynamics=OVERDAMPED)
     return sc
 
 
+def radius_mix(n=4000, density=0.5, seed=8, dt=0.002, cutoff=2.5, dpd=True, tol=None):
+    """Types of different radii and radius-dependent potentials (shifted and
+    scaled distance transforms).  tol None keeps every potential's reference
+    default tolerance."""
+    # a jittered cubic lattice (m^3 >= n sites, the first n taken), so that no
+    # pair starts deep inside a repulsive core
+    m = int(math.ceil(n ** (1.0 / 3.0) - 1e-9))
+    a = (1.0 / density) ** (1.0 / 3.0)
+    L = m * a
+    cells = _cells_for(L, cutoff)
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), axis=-1).reshape(-1, 3)[:n]
+    x = np.mod((g + 0.5) * a + rng.uniform(-0.15 * a, 0.15 * a, size=(n, 3)), L)
+    ptype = rng.integers(0, 3, size=n).astype(np.int32)
+    types = [dict(name="R%d" % t, mass=1.0 + t, radius=r, dynamics=NEWTONIAN)
+             for t, r in enumerate((0.3, 0.45, 0.6))]
+    morse = dict(kind="morse", d=1.0, a=6.0, r0=0.0, min=0.0001, max=3.0, tol=0.001, shifted=True)
+    glj = dict(kind="glj", e=1.0, m=3.0, n=6.0, k=0.0, r0=1.0, min=0.05, max=3.0, tol=0.01, shifted=False)
+    sphere = dict(kind="overlapping_sphere", mu=1.0, kc=1.0, kh=0.0, r0=0.0, min=0.001, max=10.0, tol=0.001)
+    pots = [morse, glj, sphere]
+    if tol is not None:   # the overlapping sphere's log core needs > 3048 intervals below 1e-5
+        for p in pots:
+            p["tol"] = max(tol, 1e-5) if p is sphere else tol
+    bind = [(0, 0, 0), (0, 0, 1), (1, 1, 1), (2, 1, 2), (2, 2, 2)]
+    if dpd:
+        pots.append(dict(kind="dpd", alpha=10.0, gamma=1.0, sigma=1.0, cutoff=1.0, shifted=True))
+        bind.append((3, 0, 2))
+    else:
+        bind.append((0, 0, 2))
+    return dict(name="radius_mix", dim=[L] * 3, cells=[cells] * 3, cutoff=cutoff, dt=dt, nspecies=0,
+                types=types, pots=pots, bind=bind, fluxes=[], x=x, v=np.zeros((n, 3)), ptype=ptype,
+                species=None, bonds=None, angles=None, dpd_noise=None)
+
+
 def ensemble(nrep=512, seed=5, nmin=500, nmax=20000, dt=0.005):
     rng = np.random.default_rng(seed)
     ns = np.rint(np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=nrep))).astype(int)
@@ -185,6 +225,12 @@ def make_potential(p):
     if kind == "dpd":
         return Potential.dpd(alpha=p["alpha"], gamma=p["gamma"], sigma=p["sigma"], cutoff=p["cutoff"],
                              shifted=p["shifted"])
+    if kind == "glj":
+        return Potential.glj(p["e"], m=p["m"], n=p["n"], k=p["k"], r0=p["r0"], min=p["min"], max=p["max"],
+                             tol=p["tol"], shifted=p["shifted"])
+    if kind == "overlapping_sphere":
+        return Potential.overlapping_sphere(mu=p["mu"], kc=p["kc"], kh=p["kh"], r0=p["r0"], min=p["min"],
+                                            max=p["max"], tol=p["tol"])
     raise ValueError("unknown potential kind %r" % kind)
 
 
