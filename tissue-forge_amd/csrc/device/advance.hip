@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                 b.x = xbv[0];
                 b.y = xbv[1];
                 b.z = xbv[2];
-                b.w = as_real_bits((info & 0xffff) * kPairVecs * static_cast<int>(sizeof(v4)), real(0));
+                b.w = as_real_bits(type_column<v4>(info & 0xffff), real(0));
                 P.xb[i] = b;
             }
             // Particles that changed cell skip this step's integration and keep

@@ -196,7 +196,6 @@ __global__ __launch_bounds__(64) void k_cellorder(const int *start, const int2 *
 template <typename real>
 __global__ void k_permute(KP<real> P) {
     using v4 = typename V<real>::v4;
-    constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
     const int S = P.S;
     if (P.err[0] != 0) return;   // halted
     const bool any = !P.list_mode || P.lflags[0] != 0;
@@ -220,7 +219,7 @@ __global__ void k_permute(KP<real> P) {
             v4 p = P.A.pos[i];
             P.x0[i] = p;
             P.cid0[i] = m.z;
-            p.w = as_real_bits((m.x & 0xffff) * kPairBytes, real(0));
+            p.w = as_real_bits(type_column<v4>(m.x & 0xffff), real(0));
             P.xb[i] = p;
         }
         if (P.A.p0) {   // Runge-Kutta stage state

@@ -74,6 +74,11 @@ constexpr int kListSlotBits = 25;
 constexpr int kListSlotMask = (1 << kListSlotBits) - 1;
 __host__ __device__ inline int list_code(int ox, int oy, int oz) { return ((ox + 2) * 5 + (oy + 2)) * 5 + (oz + 2); }
 constexpr int kCentreCode = 62;        // list_code(0, 0, 0)
+// The list pair pass reads entries in batches of kListBatch; the build pads
+// each list to a multiple of it with entries naming the slot np, a dummy
+// record far from every particle (its pair is outside any cutoff).
+constexpr int kListBatch = 4;
+constexpr float kListFar = 1.0e6f;
 constexpr int kMaxBuildCells = 128;    // build stencil cells (at most 5^3)
 // Entry k of particle i: tiles of 64 particles, entry-major inside a tile, so
 // that a wave of consecutive particles reads one entry index as 256
@@ -81,8 +86,13 @@ constexpr int kMaxBuildCells = 128;    // build stencil cells (at most 5^3)
 __host__ __device__ inline size_t list_index(int k, int i, int lcap) {
     return (static_cast<size_t>(i >> 6) * lcap + k) * 64 + (i & 63);
 }
-// Pair descriptors: vectors per type pair (the pair kernels' LDS table).
-constexpr int kPairVecs = 4;
+// Pair descriptors in LDS: kPairVecs fields, each an array of kDescPairs
+// vectors (one per type pair, ti * ntypes + tj), so that lanes reading the
+// descriptors of different type pairs spread over the LDS banks.  A particle
+// carries its type as the byte offset of its column, type * sizeof(v4).
+constexpr int kPairVecs = 3;
+constexpr int kDescPairs = kMaxTypes * kMaxTypes;
+template <typename v4> __host__ __device__ constexpr int type_column(int type) { return type * static_cast<int>(sizeof(v4)); }
 
 template <typename real> struct PotDesc {
     int kind;
@@ -124,9 +134,12 @@ template <typename real> struct PArrays {
 // bytes from off_waves; inside a region positions at 0 (chunk + pad
 // entries) and ids, slots (FP32 cutoff guard), velocities, radii and
 // species at their offsets.
+// Tables cached in LDS are split by halves: the first four coefficients of
+// every interval (centre, 1/half width, c2, c3) from off_coef, the last four
+// coef_half bytes further, so that random intervals spread over the banks.
 struct NbLayout {
     int waves, chunk;
-    int off_coef, off_waves, region;
+    int off_coef, coef_half, off_waves, region;
     int off_id, off_slot, off_vel, off_rad, off_sp;
     int total;
 };

@@ -459,7 +459,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     for (const SysDesc &d : sys) max_nt = std::max(max_nt, d.ntypes);
     // Lists carry tables only (no DPD, flux or radius transforms), with every
     // system's tables in LDS, and slots below 2^25.
-    E->list_mode = list_ok && !E->has_dpd && !E->has_flux && !E->has_radius && np < (1 << kListSlotBits) &&
+    E->list_mode = list_ok && !E->has_dpd && !E->has_flux && !E->has_radius && np + 1 < (1 << kListSlotBits) &&
                    static_cast<size_t>(max_coef) * sizeof(real) <= 16384 && max_coef > 0;
     E->lcap = E->list_mode ? lcap : 0;
     int rc = nonbond_setup<real>(E, max_nt, max_coef);
@@ -589,11 +589,12 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     ALLOC(E->d_lflags, int, 4);
     ALLOC(E->d_sflags, int, 3 * nsys);
     if (E->list_mode) {
-        ALLOC(E->d_list, int, (size_t)((np + 63) / 64) * 64 * E->lcap);
+        // one batch of slack: the list pass reads the batch after a list's end
+        ALLOC(E->d_list, int, (size_t)((np + 63) / 64) * 64 * E->lcap + 64 * kListBatch);
         ALLOC(E->d_lcount, int, np);
         ALLOC(E->d_cid0, int, np);
         ALLOC(E->d_lblock, int4, lblocks.size());
-        ALLOC(E->xb, v4, np);
+        ALLOC(E->xb, v4, np + 1);   // slot np: the dummy that pads the lists
         ALLOC(E->x0, v4, np);
     }
 #undef ALLOC
@@ -625,6 +626,12 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     if ((rc = H2D(E->d_bonds, hb.data(), sizeof(int4) * hb.size()))) return rc;
     if ((rc = H2D(E->d_angles, ha.data(), sizeof(int4) * ha.size()))) return rc;
     if (E->list_mode && (rc = H2D(E->d_lblock, lblocks.data(), sizeof(int4) * lblocks.size()))) return rc;
+    if (E->list_mode) {
+        v4 far;
+        far.x = far.y = far.z = real(kListFar);
+        far.w = real(0);   // bit pattern 0: type 0's descriptor column
+        if ((rc = H2D(static_cast<v4 *>(E->xb) + np, &far, sizeof(v4)))) return rc;
+    }
     {
         const int flags[4] = {0, 0, 0, 0};
         if ((rc = H2D(E->d_lflags, flags, sizeof(flags)))) return rc;

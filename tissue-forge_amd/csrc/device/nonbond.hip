@@ -43,17 +43,20 @@ constexpr int kPad = 16;                 // staged entries read past the end by 
 constexpr int kCoefLdsMax = 16384;       // table bytes cached in LDS
 constexpr int kWaveLdsBudget = 40960;    // staging bytes of a workgroup
 
-// Pair descriptor in LDS, kPairVecs vectors per type pair:
+// Pair descriptor in LDS: fields q0, q1, q2 of type pair t at vectors t,
+// kDescPairs + t and 2 kDescPairs + t (the field arrays keep lanes that read
+// different type pairs on different banks):
 //   table: q0 = (a, b, alpha0, alpha1), q1 = (alpha2, n, table bits, meta bits)
 //   DPD:   q0 = (a, b, alpha, gamma),    q1 = (sigma, 1/b, 0, meta bits)
 //   both:  q2 = (r0+1, b_hi, b_lo, 0) with b = b_hi + b_lo to 48 bits (the FP64 guard)
 // An absent pair has a = 0, b = -1, so that no distance is in range.  The
-// table bits are the LDS byte offset of the table (from the start of the
-// dynamic LDS) when the tables are cached, else its index in the global
-// coefficient array.  meta = kind << 16 | shifted << 20 | scaled << 21 |
-// b_inside << 22 (b is below the cutoff, so its end needs the guard too).
-// Staged particles carry their type as the byte offset of its descriptor
-// column (type * kPairBytes), the i particle its row (type * ntypes * kPairBytes).
+// table bits are the LDS byte offset of the table's first interval (from the
+// start of the dynamic LDS, in the first-half array) when the tables are
+// cached, else its index in the global coefficient array.  meta = kind << 16
+// | shifted << 20 | scaled << 21 | b_inside << 22 (b is below the cutoff, so
+// its end needs the guard too).  Staged particles carry their type as the
+// byte offset of its descriptor column (type_column), the i particle its row
+// (ntypes * type_column(type)).
 constexpr int kMetaDpd = 1 << 16;
 constexpr int kMetaShifted = 1 << 20;
 constexpr int kMetaScaled = 1 << 21;
@@ -115,7 +118,9 @@ NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, b
     const bool slots = std::is_same<real, float>::value || build;
     NbLayout L{};
     L.waves = kNbWaves;
-    L.off_coef = align16(max_nt * max_nt * kPairVecs * static_cast<int>(sizeof(v4)));
+    (void)max_nt;
+    L.off_coef = kPairVecs * kDescPairs * static_cast<int>(sizeof(v4));
+    L.coef_half = coef_lds_count / 2 * static_cast<int>(sizeof(real));   // a whole number of 4-vectors
     L.off_waves = L.off_coef + align16(coef_lds_count * static_cast<int>(sizeof(real)));
     if (build) L.off_waves += align16(125 * static_cast<int>(sizeof(v4)));   // shifts of the 125 build-cell offsets
     auto region = [&](int chunk, NbLayout *o) {
@@ -173,16 +178,29 @@ __device__ __forceinline__ void write_pair_descriptor(const KP<real> &P, const S
             q0.w = pd.al1;
             q1.x = pd.al2;
             q1.y = real(pd.n);
-            q1.z = as_real_bits(CLDS ? off_coef + (pd.coff - sd.coef_begin) * static_cast<int>(sizeof(real)) : pd.coff,
+            // LDS: byte offset of the table's interval 0 in the first-half array
+            q1.z = as_real_bits(CLDS ? off_coef + (pd.coff - sd.coef_begin) / 2 * static_cast<int>(sizeof(real)) : pd.coff,
                                 real(0));
         }
         q2.x = pd.r0p1;
     }
     q1.w = as_real_bits(meta, real(0));
-    s_pair[kPairVecs * t] = q0;
-    s_pair[kPairVecs * t + 1] = q1;
-    s_pair[kPairVecs * t + 2] = q2;
-    s_pair[kPairVecs * t + 3] = q3;
+    (void)q3;
+    s_pair[t] = q0;
+    s_pair[kDescPairs + t] = q1;
+    s_pair[2 * kDescPairs + t] = q2;
+}
+
+// Copies a system's table coefficients into LDS split by halves: the first
+// four of interval k to first[4 k ..], the last four to first + half bytes.
+template <typename real>
+__device__ __forceinline__ void cache_tables(const KP<real> &P, const SysDesc &sd, unsigned char *first, int half) {
+    real *lo = reinterpret_cast<real *>(first);
+    real *hi = reinterpret_cast<real *>(first + half);
+    for (int t = threadIdx.x; t < sd.coef_count; t += kNbThreads) {
+        const int k = t >> 3, j = t & 7;
+        (j < 4 ? lo : hi)[4 * k + (j & 3)] = P.coef[sd.coef_begin + t];
+    }
 }
 
 constexpr int kStageBatch = 4;   // list build: staging passes whose loads are issued together
@@ -199,7 +217,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
     constexpr bool kIds = DPD || FLUX;   // pair terms that need the partner's id
     constexpr bool kGuarded = std::is_same<real, float>::value && !BUILD;
     if (BUILD && P.lflags[0] == 0) return;
-    constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
+    constexpr int kCol = type_column<v4>(1);   // bytes per descriptor column
     extern __shared__ __align__(16) unsigned char s_dyn[];
     // A halted ensemble does nothing.  The list build raises the halt itself
     // (overflow), so the word is read once per workgroup and shared: every
@@ -216,8 +234,6 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
     const int S = P.S;
     const int chunk = P.nb.chunk;
     v4 *s_pair = reinterpret_cast<v4 *>(s_dyn);
-    real *s_coef = reinterpret_cast<real *>(s_dyn + P.nb.off_coef);
-    (void)s_coef;
     unsigned char *region = s_dyn + P.nb.off_waves + wid * P.nb.region;
     v4 *s_pos = reinterpret_cast<v4 *>(region);
     int *s_id = reinterpret_cast<int *>(region + P.nb.off_id);
@@ -259,8 +275,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
         }
         if (FLUX)
             for (int t = threadIdx.x; t < sd.nflux; t += kNbThreads) s_flux[t] = P.flux[sd.flux_first + t];
-        if (CLDS)
-            for (int t = threadIdx.x; t < sd.coef_count; t += kNbThreads) s_coef[t] = P.coef[sd.coef_begin + t];
+        if (CLDS) cache_tables(P, sd, s_dyn + P.nb.off_coef, P.nb.coef_half);
         if (BUILD)   // shift of each build-cell offset code, in the working precision
             for (int t = threadIdx.x; t < 125; t += kNbThreads) {
                 v4 sh;
@@ -363,7 +378,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                 const v4 pi4 = P.A.pos[si];
                 const int4 mi = P.A.meta[si];
                 const int ti = mi.x & 0xffff;
-                const int ti_row = ti * nt * kPairBytes;
+                const int ti_row = ti * nt * kCol;
                 v4 vi4;
                 if (DPD) vi4 = P.A.vel[si];
                 const int id_i = mi.y - sd.p_off;
@@ -403,7 +418,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     const int tpb = ti_row + real_bits_as_int(pj.w);   // descriptor byte offset
                     const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + tpb);
                     const v4 q0 = qd[0];
-                    const v4 q1 = qd[1];
+                    const v4 q1 = qd[kDescPairs];
                     const int meta = real_bits_as_int(q1.w);
                     // distance after the radius transform of the pair's potential
                     real rt = r_raw;
@@ -411,7 +426,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                         if ((meta & kMetaScaled) && !(meta & kMetaDpd))
                             rt = r_raw / (pi4.w + s_rad[k]);
                         else if (meta & kMetaShifted)
-                            rt = r_raw - (pi4.w + s_rad[k]) + ((meta & kMetaDpd) ? real(0) : qd[2].x);
+                            rt = r_raw - (pi4.w + s_rad[k]) + ((meta & kMetaDpd) ? real(0) : qd[2 * kDescPairs].x);
                     }
                     if (kGuarded && kMode == kFree) {
                         bool band = r2 > cut2_lo;
@@ -424,7 +439,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     int id_j = 0;
                     if (kIds) id_j = s_id[k] - sd.p_off;
                     if (FLUX && ok) {
-                        const int tj = real_bits_as_int(pj.w) / kPairBytes;
+                        const int tj = real_bits_as_int(pj.w) / kCol;
                         const int2 fm = s_fmap[ti * nt + tj];
                         for (int f = 0; f < fm.y; f++) {
                             const FluxDesc<real> &fd = s_flux[fm.x + f];
@@ -480,10 +495,9 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     const int ind = static_cast<int>(clamp_real(q0.z + r * (q0.w + r * q1.x), real(1), q1.y));
                     const int tb = real_bits_as_int(q1.z);
                     v4 c0, c1;
-                    if (CLDS) {
-                        const v4 *c4 = reinterpret_cast<const v4 *>(s_dyn + tb + ind * 8 * static_cast<int>(sizeof(real)));
-                        c0 = c4[0];
-                        c1 = c4[1];
+                    if (CLDS) {   // the two halves of interval ind
+                        c0 = *reinterpret_cast<const v4 *>(s_dyn + tb + ind * kCol);
+                        c1 = *reinterpret_cast<const v4 *>(s_dyn + tb + P.nb.coef_half + ind * kCol);
                     } else {
                         const v4 *c4 = reinterpret_cast<const v4 *>(P.coef + tb + ind * 8);
                         c0 = c4[0];
@@ -605,8 +619,8 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                                 if (k == self_k || !(r2 <= cut2)) continue;
                                 const int tpb = ti_row + real_bits_as_int(pj.w);
                                 const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + tpb);
-                                const v4 q0 = qd[0], q2 = qd[2];
-                                const int meta = real_bits_as_int(qd[1].w);
+                                const v4 q0 = qd[0], q2 = qd[2 * kDescPairs];
+                                const int meta = real_bits_as_int(qd[kDescPairs].w);
                                 bool band = r2 > cut2_lo;
                                 if (binside) {
                                     const real r_raw = r2 * rsq(fmax(r2, eps2));
@@ -694,7 +708,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                             const unsigned long long km = __ballot(keep);
                             if (keep) {
                                 const int dst = nstaged + __popcll(km & ((1ull << lane) - 1ull));
-                                p.w = as_real_bits((mv[u] & 0xffff) * kPairBytes, real(0));
+                                p.w = as_real_bits(type_column<v4>(mv[u] & 0xffff), real(0));
                                 s_pos[dst] = p;
                                 s_slot[dst] = static_cast<int>(static_cast<unsigned>(sj_c) |
                                                                (static_cast<unsigned>(code_c) << kListSlotBits));
@@ -777,7 +791,7 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                                     p.x = real(static_cast<double>(p.x) + shx);
                                     p.y = real(static_cast<double>(p.y) + shy);
                                     p.z = real(static_cast<double>(p.z) + oz * sd.h[2]);
-                                    p.w = as_real_bits((mj.x & 0xffff) * kPairBytes, real(0));
+                                    p.w = as_real_bits(type_column<v4>(mj.x & 0xffff), real(0));
                                     s_pos[nstaged + t] = p;
                                     if (kIds) s_id[nstaged + t] = mj.y;
                                     if (kGuarded) s_slot[nstaged + t] = sj;
@@ -810,6 +824,11 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     if (active && sl == 0) {
                         P.lcount[si] = lbase;
                         if (lbase > P.lcap) atomicOr(P.err, kHaltOverflow);
+                        // pad to whole batches of the list pass with the dummy slot
+                        const unsigned pad = static_cast<unsigned>(P.np) |
+                                             (static_cast<unsigned>(kCentreCode) << kListSlotBits);
+                        const int padded = min((lbase + kListBatch - 1) & ~(kListBatch - 1), P.lcap);
+                        for (int k = lbase; k < padded; k++) P.list[list_index(k, si, P.lcap)] = static_cast<int>(pad);
                     }
                     continue;
                 }
@@ -928,11 +947,12 @@ template <typename real> int nonbond_setup(DeviceEnsemble *E, int max_nt, int ma
     E->nonbond_resident = per_cu * E->cus;
     if (E->list_mode) {
         E->nbb = nb_layout<real>(max_nt, 0, E->S, false, false, false, max_chunk, true);
-        // list kernel: descriptors, tables, 125 build-cell offsets
+        // list kernel: descriptors and tables (split by halves)
         E->nbl = NbLayout{};
-        E->nbl.off_coef = align16(max_nt * max_nt * kPairVecs * static_cast<int>(sizeof(typename V<real>::v4)));
+        E->nbl.off_coef = kPairVecs * kDescPairs * static_cast<int>(sizeof(typename V<real>::v4));
+        E->nbl.coef_half = max_coef / 2 * static_cast<int>(sizeof(real));
         E->nbl.off_waves = E->nbl.off_coef + align16(max_coef * static_cast<int>(sizeof(real)));
-        E->nbl.total = E->nbl.off_waves + 125 * static_cast<int>(sizeof(typename V<real>::v4));
+        E->nbl.total = E->nbl.off_waves;
     }
     return TFG_OK;
 }
@@ -987,45 +1007,43 @@ __device__ inline double list_guard_r2(const KP<float> &P, const SysDesc &sd, in
 }
 __device__ inline double list_guard_r2(const KP<double> &, const SysDesc &, int, int) { return 0.0; }
 
-constexpr int kListBatch = 4;   // list entries loaded together per lane (4 measured fastest of 4, 8, 16)
 
-// Pair forces from the neighbour lists: one lane per particle, entries in
-// list order (the build's candidate order), so each particle's sum has a
-// fixed order.  Tables only (no DPD, flux or radius transforms: those
+// Pair forces from the neighbour lists: one lane per particle over its
+// entries in list order (the build's candidate order), so each particle's sum
+// has a fixed order.  Tables only (no DPD, flux or radius transforms: those
 // systems use the cell kernel).  A workgroup covers up to kNbThreads
 // particles of one system and caches its pair descriptors, tables and the
-// 125 build-cell offsets in LDS.  The pair test, the FP32 cutoff guard
-// (band pairs decided in FP64 from the stored positions), the number density
-// and the table evaluation are those of the cell kernel.
-
+// the cell-offset code of an entry is decoded arithmetically.  The pair test, the FP32 cutoff
+// guard (band pairs decided in FP64 from the stored positions), the number
+// density and the table evaluation are those of the cell kernel.
+//
+// The separation is formed in FP64 from the two build-frame positions and
+// the build-cell shift, and rounded once: adding the shift in FP32 first
+// rounds at the magnitude of a cell, several times |dx|, and was the largest
+// part of the FP32 path's per-particle force error.
+//
+// Entries are read in batches of kListBatch (loaded together, then the
+// partner positions gathered together, then evaluated in list order).  The
+// build pads every list to whole batches with the dummy slot np, so each lane
+// runs its own number of batches with no per-entry test of the list end.
 template <typename real, bool ENERGY>
 __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     using v4 = typename V<real>::v4;
     constexpr bool kGuarded = std::is_same<real, float>::value;
-    constexpr int kPairBytes = kPairVecs * static_cast<int>(sizeof(v4));
+    constexpr int kCol = type_column<v4>(1);   // bytes per descriptor column
     extern __shared__ __align__(16) unsigned char s_dyn[];
+    if (P.err[0] != 0) return;   // halted (nothing writes the word during this kernel)
     // Workgroups go to the 8 XCDs in turn: the workgroups of one XCD take
     // consecutive particle blocks (the grid is a multiple of 8), so that the
     // partner positions they gather meet in that XCD's L2.
-    if (P.err[0] != 0) return;   // halted (nothing writes the word during this kernel)
     const int nblk = gridDim.x;
     const int b = (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8;
     const int4 lb = P.lblock[b];
     const SysDesc &sd = P.sys[lb.x];
     const int nt = sd.ntypes;
     v4 *s_pair = reinterpret_cast<v4 *>(s_dyn);
-    real *s_coef = reinterpret_cast<real *>(s_dyn + P.nb.off_coef);
-    v4 *s_sh = reinterpret_cast<v4 *>(s_dyn + P.nb.off_waves);
     for (int t = threadIdx.x; t < nt * nt; t += kNbThreads) write_pair_descriptor<real, true>(P, sd, t, P.nb.off_coef, s_pair);
-    for (int t = threadIdx.x; t < sd.coef_count; t += kNbThreads) s_coef[t] = P.coef[sd.coef_begin + t];
-    for (int c = threadIdx.x; c < 125; c += kNbThreads) {
-        v4 sh;
-        sh.x = real((c / 25 - 2) * sd.h[0]);
-        sh.y = real((c / 5 % 5 - 2) * sd.h[1]);
-        sh.z = real((c % 5 - 2) * sd.h[2]);
-        sh.w = real(0);
-        s_sh[c] = sh;
-    }
+    cache_tables(P, sd, s_dyn + P.nb.off_coef, P.nb.coef_half);
     __syncthreads();
 
     const bool valid = static_cast<int>(threadIdx.x) < lb.z;
@@ -1039,49 +1057,63 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     const real eps2 = eps_of<real>() * eps_of<real>();
 
     const v4 pi = P.xb[i];
+    const double pxd = static_cast<double>(pi.x), pyd = static_cast<double>(pi.y), pzd = static_cast<double>(pi.z);
     const int ti = P.A.meta[i].x & 0xffff;
-    const int ti_row = ti * nt * kPairBytes;
+    const int ti_row = ti * nt * kCol;
     const int n = valid ? min(P.lcount[i], P.lcap) : 0;
+    const int nbat = (n + kListBatch - 1) & ~(kListBatch - 1);   // the build padded the list to here
     real fx = 0, fy = 0, fz = 0, nd = 0, ep = 0, vir = 0;
-    // Entries in batches of kListBatch: the batch's entries are loaded
-    // together, then its partner positions are gathered together, then the
-    // pairs are evaluated in list order (so many loads are in flight per lane
-    // and the sum keeps its order).
     const int *lp = P.list + list_index(0, i, P.lcap);   // entry k at lp[64 k]
-    for (int k0 = 0; __any(k0 < n); k0 += kListBatch) {
+    // The entries of the next batch are loaded while this batch computes (they
+    // stream from HBM); the partner positions of a batch are gathered at its
+    // start.  The loads of the next batch are unconditional, so that the wait
+    // for a batch's gathers never includes them: after a list's last batch
+    // they read one batch past it, inside the buffer (the allocation has one
+    // batch of slack), and are not used.  Entries are read once per step
+    // (non-temporal: L2 stays with the gathers).
+    unsigned en[kListBatch];
+#pragma unroll
+    for (int u = 0; u < kListBatch; u++) en[u] = static_cast<unsigned>(__builtin_nontemporal_load(lp + 64 * u));
+    for (int k0 = 0; k0 < nbat; k0 += kListBatch) {
         unsigned ev[kListBatch];
         v4 pv[kListBatch];
-        // entries past the list are read from inside the allocation (index
-        // clamped to the capacity) and replaced by the particle itself, so the
-        // loads need no branch and no gather leaves the buffers; entries are
-        // read once per step (non-temporal: L2 stays with the gathers)
 #pragma unroll
-        for (int u = 0; u < kListBatch; u++) {
-            const unsigned raw = static_cast<unsigned>(__builtin_nontemporal_load(lp + 64 * min(k0 + u, P.lcap - 1)));
-            ev[u] = k0 + u < n ? raw : static_cast<unsigned>(i);
-        }
+        for (int u = 0; u < kListBatch; u++) ev[u] = en[u];
 #pragma unroll
         for (int u = 0; u < kListBatch; u++) pv[u] = P.xb[ev[u] & kListSlotMask];
+        // memory counters retire in issue order: the gathers go first, so that
+        // waiting for them never waits for the next batch's entries
+        __builtin_amdgcn_sched_barrier(0);
+        lp += 64 * kListBatch;
+#pragma unroll
+        for (int u = 0; u < kListBatch; u++) en[u] = static_cast<unsigned>(__builtin_nontemporal_load(lp + 64 * u));
 #pragma unroll
         for (int u = 0; u < kListBatch; u++) {
-            const bool live = k0 + u < n;
             const unsigned e = ev[u];
             const int j = static_cast<int>(e & kListSlotMask);
-            const v4 sh = s_sh[live ? (e >> kListSlotBits) : kCentreCode];
-            v4 pj = pv[u];
-            pj.x = pj.x + sh.x;
-            pj.y = pj.y + sh.y;
-            pj.z = pj.z + sh.z;
-            const real dx = pi.x - pj.x;
-            const real dy = pi.y - pj.y;
-            const real dz = pi.z - pj.z;
-            const real r2 = dist2(pi, pj);
-            bool ok = live && r2 <= cut2;
+            // build-cell offset code ((ox + 2) 5 + (oy + 2)) 5 + (oz + 2) < 125:
+            // code / 25 = (code * 2622) >> 16 and rem / 5 = (rem * 13108) >> 16
+            // hold exactly on that range
+            const int code = static_cast<int>(e >> kListSlotBits);
+            const int ox = (code * 2622) >> 16;
+            const int rem = code - 25 * ox;
+            const int oy = (rem * 13108) >> 16;
+            const int oz = rem - 5 * oy;
+            const v4 pj = pv[u];
+            const real dx = real(pxd - __builtin_fma(static_cast<double>(ox - 2), sd.h[0], static_cast<double>(pj.x)));
+            const real dy = real(pyd - __builtin_fma(static_cast<double>(oy - 2), sd.h[1], static_cast<double>(pj.y)));
+            const real dz = real(pzd - __builtin_fma(static_cast<double>(oz - 2), sd.h[2], static_cast<double>(pj.z)));
+            real r2;
+            {
+#pragma clang fp contract(off)
+                r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+            }
+            bool ok = r2 <= cut2;
             const real ir = rsq(fmax(r2, eps2));
             const real r_raw = r2 * ir;
             const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + ti_row + real_bits_as_int(pj.w));
             const v4 q0 = qd[0];
-            const v4 q1 = qd[1];
+            const v4 q1 = qd[kDescPairs];
             bool in = ok && r_raw <= q0.y;
             if (kGuarded) {
                 const int meta = real_bits_as_int(q1.w);
@@ -1089,7 +1121,7 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
                 if (binside) band = band || (ok && (meta & kMetaBInside) && fabs(r_raw - q0.y) <= real(2.0 * kGuard) * q0.y);
                 if (__builtin_expect(band, false)) {
                     const double r2d = list_guard_r2(P, sd, i, j);
-                    const v4 q2 = qd[2];
+                    const v4 q2 = qd[2 * kDescPairs];
                     const double bd = static_cast<double>(q2.y) + static_cast<double>(q2.z);
                     ok = r2d <= sd.cutoff2;
                     in = ok && r2d <= bd * bd;
@@ -1099,9 +1131,8 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
             const real r = fmax(r_raw, q0.x);
             const int ind = static_cast<int>(clamp_real(q0.z + r * (q0.w + r * q1.x), real(1), q1.y));
             const int tb = real_bits_as_int(q1.z);
-            const v4 *c4 = reinterpret_cast<const v4 *>(s_dyn + tb + ind * 8 * static_cast<int>(sizeof(real)));
-            const v4 c0 = c4[0];
-            const v4 c1 = c4[1];
+            const v4 c0 = *reinterpret_cast<const v4 *>(s_dyn + tb + ind * kCol);
+            const v4 c1 = *reinterpret_cast<const v4 *>(s_dyn + tb + P.nb.coef_half + ind * kCol);
             const real x = (r - c0.x) * c0.y;
             real ee = c0.z * x + c0.w;
             real eff = c0.z;
