@@ -589,8 +589,9 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     ALLOC(E->d_lflags, int, 4);
     ALLOC(E->d_sflags, int, 3 * nsys);
     if (E->list_mode) {
-        // one batch of slack: the list pass reads the batch after a list's end
-        ALLOC(E->d_list, int, (size_t)((np + 63) / 64) * 64 * E->lcap + 64 * kListBatch);
+        // slack for the list pass, which reads the batch after a list's end
+        // (room for batches of up to 16 entries)
+        ALLOC(E->d_list, int, (size_t)((np + 63) / 64) * 64 * E->lcap + 64 * 16);
         ALLOC(E->d_lcount, int, np);
         ALLOC(E->d_cid0, int, np);
         ALLOC(E->d_lblock, int4, lblocks.size());

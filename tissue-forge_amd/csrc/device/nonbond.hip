@@ -42,6 +42,9 @@ constexpr int kNbThreads = kNbWaves * kWave;
 constexpr int kPad = 16;                 // staged entries read past the end by the unrolled test
 constexpr int kCoefLdsMax = 16384;       // table bytes cached in LDS
 constexpr int kWaveLdsBudget = 40960;    // staging bytes of a workgroup
+// Byte distance between the (x, y) and the (z, 0) arrays of the list pass's
+// shift table (a constant, so the second read takes an immediate offset).
+constexpr int kListShiftZ = 2048;
 
 // Pair descriptor in LDS: fields q0, q1, q2 of type pair t at vectors t,
 // kDescPairs + t and 2 kDescPairs + t (the field arrays keep lanes that read
@@ -952,7 +955,7 @@ template <typename real> int nonbond_setup(DeviceEnsemble *E, int max_nt, int ma
         E->nbl.off_coef = kPairVecs * kDescPairs * static_cast<int>(sizeof(typename V<real>::v4));
         E->nbl.coef_half = max_coef / 2 * static_cast<int>(sizeof(real));
         E->nbl.off_waves = E->nbl.off_coef + align16(max_coef * static_cast<int>(sizeof(real)));
-        E->nbl.total = E->nbl.off_waves;
+        E->nbl.total = E->nbl.off_waves + kListShiftZ + 125 * 16;   // the FP64 build-cell shifts
     }
     return TFG_OK;
 }
@@ -981,6 +984,9 @@ template <typename real> int run_list_build(DeviceEnsemble *E, const KP<real> &P
     return launch_error("list build");
 }
 
+
+__device__ inline float fma_of(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_of(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // FP64 squared distance of particles i and j from their stored positions and
 // current cells (the list kernel's cutoff guard): the cell offset is taken
@@ -1013,7 +1019,7 @@ __device__ inline double list_guard_r2(const KP<double> &, const SysDesc &, int,
 // has a fixed order.  Tables only (no DPD, flux or radius transforms: those
 // systems use the cell kernel).  A workgroup covers up to kNbThreads
 // particles of one system and caches its pair descriptors, tables and the
-// the cell-offset code of an entry is decoded arithmetically.  The pair test, the FP32 cutoff
+// the 125 build-cell shifts (in FP64) in LDS.  The pair test, the FP32 cutoff
 // guard (band pairs decided in FP64 from the stored positions), the number
 // density and the table evaluation are those of the cell kernel.
 //
@@ -1042,8 +1048,14 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     const SysDesc &sd = P.sys[lb.x];
     const int nt = sd.ntypes;
     v4 *s_pair = reinterpret_cast<v4 *>(s_dyn);
+    // build-cell shifts by offset code: (x, y) at 16 code, (z, 0) kListShiftZ bytes further
+    double2 *s_sh = reinterpret_cast<double2 *>(s_dyn + P.nb.off_waves);
     for (int t = threadIdx.x; t < nt * nt; t += kNbThreads) write_pair_descriptor<real, true>(P, sd, t, P.nb.off_coef, s_pair);
     cache_tables(P, sd, s_dyn + P.nb.off_coef, P.nb.coef_half);
+    for (int c = threadIdx.x; c < 125; c += kNbThreads) {
+        s_sh[c] = make_double2((c / 25 - 2) * sd.h[0], (c / 5 % 5 - 2) * sd.h[1]);
+        s_sh[kListShiftZ / 16 + c] = make_double2((c % 5 - 2) * sd.h[2], 0.0);
+    }
     __syncthreads();
 
     const bool valid = static_cast<int>(threadIdx.x) < lb.z;
@@ -1091,23 +1103,14 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
         for (int u = 0; u < kListBatch; u++) {
             const unsigned e = ev[u];
             const int j = static_cast<int>(e & kListSlotMask);
-            // build-cell offset code ((ox + 2) 5 + (oy + 2)) 5 + (oz + 2) < 125:
-            // code / 25 = (code * 2622) >> 16 and rem / 5 = (rem * 13108) >> 16
-            // hold exactly on that range
-            const int code = static_cast<int>(e >> kListSlotBits);
-            const int ox = (code * 2622) >> 16;
-            const int rem = code - 25 * ox;
-            const int oy = (rem * 13108) >> 16;
-            const int oz = rem - 5 * oy;
+            const unsigned char *shp = reinterpret_cast<const unsigned char *>(s_sh) + 16 * (e >> kListSlotBits);
+            const double2 shxy = *reinterpret_cast<const double2 *>(shp);
+            const double shz = *reinterpret_cast<const double *>(shp + kListShiftZ);
             const v4 pj = pv[u];
-            const real dx = real(pxd - __builtin_fma(static_cast<double>(ox - 2), sd.h[0], static_cast<double>(pj.x)));
-            const real dy = real(pyd - __builtin_fma(static_cast<double>(oy - 2), sd.h[1], static_cast<double>(pj.y)));
-            const real dz = real(pzd - __builtin_fma(static_cast<double>(oz - 2), sd.h[2], static_cast<double>(pj.z)));
-            real r2;
-            {
-#pragma clang fp contract(off)
-                r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-            }
+            const real dx = real(pxd - (static_cast<double>(pj.x) + shxy.x));
+            const real dy = real(pyd - (static_cast<double>(pj.y) + shxy.y));
+            const real dz = real(pzd - (static_cast<double>(pj.z) + shz));
+            const real r2 = fma_of(dz, dz, fma_of(dy, dy, dx * dx));
             bool ok = r2 <= cut2;
             const real ir = rsq(fmax(r2, eps2));
             const real r_raw = r2 * ir;
@@ -1128,7 +1131,7 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
                 }
             }
             nd += ok ? w0 + r2 * (w2 + r_raw * w3) : real(0);
-            const real r = fmax(r_raw, q0.x);
+            const real r = r_raw > q0.x ? r_raw : q0.x;   // the table's left end (a select: no NaN is possible here)
             const int ind = static_cast<int>(clamp_real(q0.z + r * (q0.w + r * q1.x), real(1), q1.y));
             const int tb = real_bits_as_int(q1.z);
             const v4 c0 = *reinterpret_cast<const v4 *>(s_dyn + tb + ind * kCol);
