@@ -41,6 +41,9 @@ typedef struct tfg_ensemble tfg_ensemble;
 /* Integrators (source/mdcore/include/tfEngine.h:89-92). */
 #define TFG_FORWARD_EULER 0
 #define TFG_RUNGE_KUTTA_4 1
+/* Sum rules of tfg_engine_set_sum_rule. */
+#define TFG_SUM_FIRST_IN_RANGE 0
+#define TFG_SUM_ALL 1
 /* Boundary condition kinds (BoundaryConditionKind,
  * source/mdcore/include/tfBoundaryConditions.h:38-46) and faces. */
 #define TFG_BOUNDARY_VELOCITY 1u
@@ -84,6 +87,15 @@ int tfg_engine_set_max_distance_fraction(tfg_engine *e, double fraction);
  * TFG_E_INVALIDARG, as the reference rejects it. */
 int tfg_engine_set_integrator(tfg_engine *e, int kind);
 int tfg_engine_integrator(const tfg_engine *e);
+/* How a sum bound as a pair potential adds up: TFG_SUM_FIRST_IN_RANGE, the
+ * CPU engine's potential_eval_super_ex (source/mdcore/src/tf_potential_eval.h:
+ * 211-215: pca || pcb, so the first constituent whose range holds the pair
+ * is the only one evaluated), or TFG_SUM_ALL, the reference's GPU path
+ * (potential_eval_super_ex_cuda, source/mdcore/src/tfRunner_cuda.cu:1432-1474,
+ * every constituent packed by engine_cuda_build_pots_pack, :436-550, each in
+ * range adding its term).  Default TFG_SUM_FIRST_IN_RANGE. */
+int tfg_engine_set_sum_rule(tfg_engine *e, int rule);
+int tfg_engine_sum_rule(const tfg_engine *e);
 /* One face's boundary condition: BoundaryCondition::init
  * (source/mdcore/src/tfBoundaryConditions.cpp:151-171; NO_SLIP becomes a
  * velocity face with zero velocity), restore and the wall radius
@@ -207,6 +219,32 @@ int tfg_potential_coeffs(const tfg_potential *p, double *out, int cap);
  * source/mdcore/src/tf_potential_eval.h:104-146): energy and force/r. */
 int tfg_potential_eval(const tfg_potential *p, double r, double ri, double rj, double *e,
                        double *f_over_r);
+/* Potential::operator+ (source/mdcore/src/tfPotential.cpp:2706-2727): the sum
+ * of a and b, flattened into its constituents as Potential::constituents
+ * (:2684-2704) lists them (a's, then b's).  Fails with TFG_E_INVALIDARG when
+ * a is an angle potential and b is not ("Incompatible potentials") or when
+ * the sum would hold more than 7 constituents (a base and its images on
+ * both sides of three periodic axes, as create_periodic_potential makes
+ * them at order 1, tfPotential.cpp:158-179).  Constituents are tables or
+ * DPD; a and b are copied, so either may be destroyed afterwards.  How the
+ * pair pass adds the constituents up is the engine's sum rule
+ * (tfg_engine_set_sum_rule). */
+int tfg_potential_sum(tfg_potential **out, const tfg_potential *a, const tfg_potential *b);
+/* The periodic image offset of a potential (Potential::offset with
+ * POTENTIAL_PERIODIC, the record _do_create_periodic_potential writes,
+ * source/mdcore/src/tfPotential.cpp:136-155): the pair is evaluated at
+ * dx - offset (tf_potential_eval.h:220-225; tfRunner_cuda.cu:1381-1391),
+ * with dx taken from the particle of lower id to the other.  (The reference
+ * takes dx in its traversal order; the two agree whenever the result does
+ * not depend on that order, as for the mirrored images the reference
+ * creates, evaluated under TFG_SUM_ALL.)  Sets the flag; a sum takes offsets
+ * per constituent (TFG_E_INVALIDARG). */
+int tfg_potential_set_offset(tfg_potential *p, const double offset[3]);
+int tfg_potential_offset(const tfg_potential *p, double offset[3]);
+/* Constituent count of a sum (0 for a single potential) and a copy of
+ * constituent k (destroy it with tfg_potential_destroy). */
+int tfg_potential_terms(const tfg_potential *p);
+int tfg_potential_term(const tfg_potential *p, int k, tfg_potential **out);
 
 /* ---- bindings: engine_addpot (source/mdcore/src/tfEngine.cpp:1054-1077). ---- */
 int tfg_engine_addpot(tfg_engine *e, const tfg_potential *p, int type_i, int type_j);

@@ -16,6 +16,10 @@
 //   runner_dosort + sort_descending ....... src/tf_runner_dosort.cpp:64-126, src/tfRunner.cpp:73-136
 //   runner_dopair / runner_doself ......... src/tf_runner_dopair.cpp:72-332, 514-758
 //   potential_eval_ex / _super_ex ......... src/tf_potential_eval.h:43-146, 207-272
+//   Potential::operator+, constituents .... src/tfPotential.cpp:2684-2727 (sums; both sum
+//                                           rules: tf_potential_eval.h:211-215 and
+//                                           src/tfRunner_cuda.cu:1432-1474)
+//   periodic image offsets ................ src/tfPotential.cpp:136-155, tf_potential_eval.h:220-225
 //   potential_eval / potential_eval_r ..... src/tf_potential_eval.h:74-102, 165-194
 //   dpd_eval .............................. src/tf_dpd_eval.h:32-76
 //   flux_eval_ex .......................... src/tf_flux_eval.h:32-103
@@ -41,6 +45,10 @@
 //     instead of a random direction.
 //   * Between particles of equal type, the flux "a" role goes to the lower id
 //     (the reference uses thread-dependent storage order).
+//   * A periodic image offset applies to the separation taken from the
+//     lower id to the higher (the reference takes it in its traversal order,
+//     which decides a lone image, or two mirrored images both in range under
+//     the first-in-range sum rule).
 //   * The DPD random term is 1 (the reference's mode) or a Philox4x32-10
 //     draw keyed by (min id, max id, step) and the seed.
 //   * Persistent forces (force_i) are zero; no clusters, no ghosts.
@@ -60,8 +68,12 @@ namespace {
 
 constexpr int kMaxS = 8;
 
-enum { POT_SCALED = 1u << 11, POT_SHIFTED = 1u << 12 };
-enum { KIND_TABLE = 0, KIND_DPD = 1 };
+enum { POT_SCALED = 1u << 11, POT_SHIFTED = 1u << 12, POT_SUM = 1u << 14, POT_PERIODIC = 1u << 15 };
+enum { KIND_TABLE = 0, KIND_DPD = 1, KIND_SUM = 3 };
+// How a sum adds up: 0 the first constituent in range (pca || pcb,
+// tf_potential_eval.h:211-215), 1 every constituent in range (the GPU path,
+// tfRunner_cuda.cu:1432-1474).
+enum { SUM_FIRST_IN_RANGE = 0, SUM_ALL = 1 };
 enum { F_FROZEN_X = 1u << 3, F_FROZEN_Y = 1u << 4, F_FROZEN_Z = 1u << 5, F_CLUSTER = 1u << 1 };
 
 struct Pot {
@@ -72,6 +84,8 @@ struct Pot {
     int n = 0;
     std::vector<double> c;
     double dalpha = 0, dgamma = 0, dsigma = 0;
+    double offset[3] = {0, 0, 0};   // periodic image (POT_PERIODIC)
+    std::vector<int> terms;         // KIND_SUM: constituents, flattened (Potential::constituents)
 };
 
 struct Flux {
@@ -193,6 +207,7 @@ struct System {
     int64_t time = 0;
     int dpd_noise = 0;
     uint64_t seed = 0;
+    int sum_rule = SUM_FIRST_IN_RANGE;
     // Pair-task sweep (tf_runner_dopair.cpp:147-163):
     //   0  no pruning: every particle pair of the two cells is tested;
     //   1  sorted-projection pruning with signed bounds (exact when the cells
@@ -339,6 +354,57 @@ void flux_pair(const System &s, double r2, Part &p1, Part &p2) {
     }
 }
 
+// One potential (not a sum) of potential_eval_super_ex
+// (tf_potential_eval.h:220-269): the separation from the periodic image the
+// potential refers to, then dpd_eval or potential_eval_ex.  True when the
+// pair is in the potential's range.
+bool term_eval(const System &s, const Pot &p, Part &pi, Part &pj, const double dx_in[3], double r2_in,
+               double &epot, double &vir) {
+    double dx[3] = {dx_in[0], dx_in[1], dx_in[2]};
+    double r2 = r2_in;
+    if (p.flags & POT_PERIODIC) {
+        // fptype_r2(dx, offset, _dx): _dx = dx - offset (tf_fptype.h:87),
+        // with the pair oriented from the lower id to the higher
+        const double sg = pi.id < pj.id ? 1.0 : -1.0;
+        for (int k = 0; k < 3; k++) dx[k] = dx_in[k] - sg * p.offset[k];
+        r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+    }
+    if (p.kind == KIND_DPD) {
+        // dpd_eval (tf_dpd_eval.h:32-76).
+        const double eps = std::numeric_limits<double>::epsilon();
+        const double delta = 1.0 / std::sqrt(s.dt);
+        double r = std::sqrt(r2);
+        const double ro = r < eps ? eps : r;
+        r = (p.flags & POT_SHIFTED) ? r - (pi.radius + pj.radius) : r;
+        if (r > p.b) return false;
+        r = r >= p.a ? r : p.a;
+        const double e[3] = {dx[0] / ro, dx[1] / ro, dx[2] / ro};
+        const double v[3] = {pi.v[0] - pj.v[0], pi.v[1] - pj.v[1], pi.v[2] - pj.v[2]};
+        const double om = r < 0.0 ? 1.0 : (1 - r / p.b);
+        const double fc = p.dalpha * om;
+        const double fd = -p.dgamma * om * om * (e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
+        const double xi = s.dpd_noise ? xi_of(uint32_t(pi.id), uint32_t(pj.id), uint64_t(s.time), s.seed) : 1.0;
+        const double fr = p.dsigma * om * delta * xi;
+        const double f = fc + fd + fr;
+        for (int k = 0; k < 3; k++) {
+            pj.f[k] = pj.f[k] - f * e[k];
+            pi.f[k] = pi.f[k] + f * e[k];
+        }
+        vir += f * r2 / ro;
+        return true;
+    }
+    double e, f;
+    if (!eval_ex(p, pi.radius, pj.radius, r2, e, f)) return false;
+    for (int k = 0; k < 3; k++) {
+        const double w = f * dx[k];
+        pi.f[k] -= w;
+        pj.f[k] += w;
+    }
+    epot += e;
+    vir -= f * r2;
+    return true;
+}
+
 // One interacting pair (the body shared by runner_doself and runner_dopair):
 // number density, flux, then potential_eval_super_ex.
 inline void pair_body(const System &s, Part &pi, Part &pj, const double dx[3], double r2, double &epot,
@@ -359,39 +425,15 @@ inline void pair_body(const System &s, Part &pi, Part &pj, const double dx[3], d
     if (has_flux) flux_pair(s, r2, pi, pj);
     if (pidx < 0) return;
     const Pot &p = s.pots[pidx];
-    if (p.kind == KIND_DPD) {
-        // dpd_eval (tf_dpd_eval.h:32-76).
-        const double eps = std::numeric_limits<double>::epsilon();
-        const double delta = 1.0 / std::sqrt(s.dt);
-        double r = std::sqrt(r2);
-        const double ro = r < eps ? eps : r;
-        r = (p.flags & POT_SHIFTED) ? r - (pi.radius + pj.radius) : r;
-        if (r > p.b) return;
-        r = r >= p.a ? r : p.a;
-        const double e[3] = {dx[0] / ro, dx[1] / ro, dx[2] / ro};
-        const double v[3] = {pi.v[0] - pj.v[0], pi.v[1] - pj.v[1], pi.v[2] - pj.v[2]};
-        const double om = r < 0.0 ? 1.0 : (1 - r / p.b);
-        const double fc = p.dalpha * om;
-        const double fd = -p.dgamma * om * om * (e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
-        const double xi = s.dpd_noise ? xi_of(uint32_t(pi.id), uint32_t(pj.id), uint64_t(s.time), s.seed) : 1.0;
-        const double fr = p.dsigma * om * delta * xi;
-        const double f = fc + fd + fr;
-        for (int k = 0; k < 3; k++) {
-            pj.f[k] = pj.f[k] - f * e[k];
-            pi.f[k] = pi.f[k] + f * e[k];
-        }
-        vir += f * r2 / ro;
+    if (p.kind != KIND_SUM) {
+        term_eval(s, p, pi, pj, dx, r2, epot, vir);
         return;
     }
-    double e, f;
-    if (!eval_ex(p, pi.radius, pj.radius, r2, e, f)) return;
-    for (int k = 0; k < 3; k++) {
-        const double w = f * dx[k];
-        pi.f[k] -= w;
-        pj.f[k] += w;
-    }
-    epot += e;
-    vir -= f * r2;
+    // potential_eval_super_ex on a sum (tf_potential_eval.h:211-215): the
+    // constituents in order, stopping at the first in range under the CPU
+    // rule, all of them under the GPU rule.
+    for (int t : p.terms)
+        if (term_eval(s, s.pots[t], pi, pj, dx, r2, epot, vir) && s.sum_rule == SUM_FIRST_IN_RANGE) break;
 }
 
 // dpd_boundary_eval (tf_dpd_eval.h:78-124): the wall as a particle of
@@ -1158,6 +1200,41 @@ int or_add_dpd(void *h, double alpha, double gamma, double sigma, double a, doub
     p.dsigma = sigma;
     s->pots.push_back(p);
     return (int)s->pots.size() - 1;
+}
+
+// Potential::operator+ (tfPotential.cpp:2706-2727) over already added
+// potentials, flattened like Potential::constituents (:2684-2704).
+int or_add_sum(void *h, int nterms, const int *ids) {
+    System *s = static_cast<System *>(h);
+    Pot p;
+    p.kind = KIND_SUM;
+    p.flags = POT_SUM;
+    for (int k = 0; k < nterms; k++) {
+        if (ids[k] < 0 || ids[k] >= (int)s->pots.size()) return -1;
+        const Pot &q = s->pots[ids[k]];
+        if (q.kind == KIND_SUM)
+            p.terms.insert(p.terms.end(), q.terms.begin(), q.terms.end());
+        else
+            p.terms.push_back(ids[k]);
+    }
+    s->pots.push_back(p);
+    return (int)s->pots.size() - 1;
+}
+
+// Potential::offset with POTENTIAL_PERIODIC (tfPotential.cpp:136-155).
+int or_set_offset(void *h, int pot, const double offset[3]) {
+    System *s = static_cast<System *>(h);
+    if (pot < 0 || pot >= (int)s->pots.size() || s->pots[pot].kind == KIND_SUM) return -1;
+    for (int k = 0; k < 3; k++) s->pots[pot].offset[k] = offset[k];
+    s->pots[pot].flags |= POT_PERIODIC;
+    return 0;
+}
+
+int or_set_sum_rule(void *h, int rule) {
+    System *s = static_cast<System *>(h);
+    if (rule != SUM_FIRST_IN_RANGE && rule != SUM_ALL) return -1;
+    s->sum_rule = rule;
+    return 0;
 }
 
 int or_bind(void *h, int pot, int ti, int tj) {

@@ -28,6 +28,9 @@ _SIG = {
                                     ctypes.c_int, _dp]),
     "or_add_dpd": (ctypes.c_int, [_vp] + [ctypes.c_double] * 5 + [ctypes.c_uint]),
     "or_bind": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "or_add_sum": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "or_set_offset": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "or_set_sum_rule": (ctypes.c_int, [_vp, ctypes.c_int]),
     "or_add_flux": (ctypes.c_int, [_vp] + [ctypes.c_int] * 5 + [ctypes.c_double] * 4),
     "or_add_particles": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp]),
     "or_set_flags": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _up]),
@@ -52,6 +55,8 @@ _SIG = {
 }
 
 _lib = None
+
+POT_PERIODIC = 1 << 15   # PotentialFlags POTENTIAL_PERIODIC
 
 
 def build():
@@ -117,17 +122,10 @@ class Oracle:
                 raise ValueError("oracle rejected a boundary condition")
         for t in scene["types"]:
             L.or_add_type(self._h, t["mass"], t["radius"], t["dynamics"])
-        ids = []
-        for pot, spec in zip(potentials, scene["pots"]):
-            info, coef = table_of(pot)
-            if info["kind"] == 1:
-                ids.append(L.or_add_dpd(self._h, spec["alpha"], spec["gamma"], spec["sigma"], info["a"], info["b"],
-                                        info["flags"]))
-            else:
-                al = np.ascontiguousarray(info["alpha"], dtype=np.float64)
-                ids.append(L.or_add_table(self._h, info["flags"], info["a"], info["b"], info["r0_plusone"], _d(al),
-                                          info["n"], _d(coef)))
-        self.pot_ids = ids
+        if L.or_set_sum_rule(self._h, int(scene.get("sum_rule", 0))) != 0:
+            raise ValueError("unknown sum rule")
+        self.pot_ids = [self._add_potential(pot, spec) for pot, spec in zip(potentials, scene["pots"])]
+        ids = self.pot_ids
         for pi, a, b in scene["bind"]:
             L.or_bind(self._h, ids[pi], a, b)
         for face, t, pi in scene.get("walls") or []:
@@ -155,6 +153,32 @@ class Oracle:
         if scene.get("angles") is not None:
             ijk = np.ascontiguousarray(scene["angles"]["ijk"], dtype=np.int32)
             L.or_add_angles(self._h, ids[scene["angles"]["pot"]], ijk.shape[0], ijk.ctypes.data_as(_ip))
+
+    def _add_potential(self, pot, spec):
+        """Adds one product Potential (its table, DPD parameters from the
+        scene's spec, periodic offset, or the constituents of a sum)."""
+        L = lib()
+        terms = pot.constituents()
+        if terms:
+            sub = np.ascontiguousarray([self._add_potential(t, ts) for t, ts in zip(terms, spec["terms"])],
+                                       dtype=np.int32)
+            pid = L.or_add_sum(self._h, sub.shape[0], sub.ctypes.data_as(_ip))
+            if pid < 0:
+                raise ValueError("oracle rejected a sum")
+            return pid
+        info, coef = table_of(pot)
+        if info["kind"] == 1:
+            pid = L.or_add_dpd(self._h, spec["alpha"], spec["gamma"], spec["sigma"], info["a"], info["b"],
+                               info["flags"])
+        else:
+            al = np.ascontiguousarray(info["alpha"], dtype=np.float64)
+            pid = L.or_add_table(self._h, info["flags"], info["a"], info["b"], info["r0_plusone"], _d(al),
+                                 info["n"], _d(coef))
+        if info["flags"] & POT_PERIODIC:
+            off = np.ascontiguousarray(pot.offset, dtype=np.float64)
+            if L.or_set_offset(self._h, pid, _d(off)) != 0:
+                raise ValueError("oracle rejected an offset")
+        return pid
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -2,6 +2,7 @@
 reports errors the way the reference's HRESULT convention does.  No device
 compute is called here."""
 import ctypes
+import math
 import os
 import re
 
@@ -164,3 +165,80 @@ def test_engine_config_mirror(tf, W):
         assert cfg.get_device() == -1
         with pytest.raises(tf.TfgError):
             cfg.to_device()
+
+
+def test_potential_sums_and_offsets(tf):
+    """Potential::operator+ (tfPotential.cpp:2706-2727): constituents
+    flattened in order; an angle potential on the left of a non-angle one is
+    refused, the other way round is allowed; at most 7 constituents; the
+    periodic offset is set per constituent and marks it periodic."""
+    P = tf.Potential
+    m1 = P.morse(d=1.0, a=3.0, r0=1.0, min=0.0001, max=1.5, shifted=False)
+    m2 = P.morse(d=0.5, a=2.0, r0=1.5, min=0.0001, max=2.8, shifted=False)
+    d = P.dpd(alpha=10.0, gamma=1.0, sigma=1.0, cutoff=1.0)
+    s = m1 + m2
+    assert [t.info()["kind"] for t in s.constituents()] == [0, 0]
+    assert s.info()["flags"] & (1 << 14)
+    s3 = s + d
+    terms = s3.constituents()
+    assert len(terms) == 3 and terms[2].info()["kind"] == 1
+    assert np.array_equal(terms[0].coeffs(), m1.coeffs()) and np.array_equal(terms[1].coeffs(), m2.coeffs())
+    nested = (m1 + d) + (m2 + m1)
+    assert [t.info()["b"] for t in nested.constituents()] == [1.5, 1.0, 2.8, 1.5]
+    ang = P.harmonic_angle(1.0, 2.0)
+    with pytest.raises(tf.TfgError, match="Incompatible|incompatible"):
+        ang + m1
+    assert len((m1 + ang).constituents()) == 2
+    big = m1
+    for _ in range(6):
+        big = big + m2
+    assert len(big.constituents()) == 7
+    with pytest.raises(tf.TfgError, match="7 constituents"):
+        big + m1
+    # sums evaluate every tabulated constituent in range (Potential::operator())
+    e, f = s(1.2)
+    e1, f1 = m1(1.2)
+    e2, f2 = m2(1.2)
+    assert e == e1 + e2 and f == f1 + f2
+    assert s(2.0) == m2(2.0)
+    # offsets
+    assert np.array_equal(m1.offset, np.zeros(3)) and not m1.info()["flags"] & (1 << 15)
+    m1.offset = (0.5, -0.25, 0.0)
+    assert np.array_equal(m1.offset, [0.5, -0.25, 0.0]) and m1.info()["flags"] & (1 << 15)
+    with pytest.raises(tf.TfgError):
+        s.offset = (1.0, 0.0, 0.0)
+    with pytest.raises(tf.TfgError):
+        m2.offset = (math.nan, 0.0, 0.0)
+    s_img = m1 + m2
+    assert s_img.constituents()[0].info()["flags"] & (1 << 15)
+    assert np.array_equal(s_img.constituents()[0].offset, [0.5, -0.25, 0.0])
+
+
+def test_sum_rule_setting(tf):
+    eng = tf.Engine(dim=[10.0] * 3, cells=[3] * 3, cutoff=2.0)
+    assert eng.sum_rule == tf.SUM_FIRST_IN_RANGE
+    eng.sum_rule = tf.SUM_ALL
+    assert eng.sum_rule == tf.SUM_ALL
+    with pytest.raises(tf.TfgError):
+        eng.sum_rule = 2
+    assert eng.sum_rule == tf.SUM_ALL
+
+
+def test_sums_refused_where_the_reference_takes_one_record(tf):
+    """Walls (boundary_eval) and bonds/angles evaluate one table record: a
+    sum or a periodic image there is refused."""
+    P = tf.Potential
+    eng = tf.Engine(dim=[10.0] * 3, cells=[3] * 3, cutoff=2.0)
+    t = eng.add_type("A")
+    m1 = P.morse(d=1.0, a=3.0, r0=1.0, min=0.0001, max=1.5, shifted=False)
+    h = P.harmonic(1.0, 1.0)
+    s = m1 + h
+    eng.set_boundary("bottom", tf.BOUNDARY_POTENTIAL | tf.BOUNDARY_FREESLIP)
+    with pytest.raises(tf.TfgError):
+        eng.set_wall_potential("bottom", t, s)
+    eng.add_particles(np.zeros(2, dtype=np.int32), np.array([[1.0, 1.0, 1.0], [1.5, 1.0, 1.0]]), np.zeros((2, 3)))
+    with pytest.raises(tf.TfgError):
+        eng.add_bonds(s, np.array([[0, 1]], dtype=np.int32))
+    h.offset = (0.1, 0.0, 0.0)
+    with pytest.raises(tf.TfgError):
+        eng.add_bonds(h, np.array([[0, 1]], dtype=np.int32))

@@ -30,15 +30,17 @@ def scenes(W, tf):
     ml = W.morse_lj(n=3000)
     ml["verlet_skin"] = 0.3
     ml["verlet_capacity"] = 96
+    sums = W.sum_mix(n=800, rule=tf.SUM_ALL)
     return {"cell_sorting": cs, "dpd_flux": dpd, "bonded_rk4": bonded, "walled": walled,
-            "couette": couette, "morse_lj_lists": ml}
+            "couette": couette, "morse_lj_lists": ml, "sums": sums}
 
 
 def _pyfloat(v):
     return float(v) if not isinstance(v, str) else {"nan": math.nan, "inf": math.inf, "-inf": -math.inf}[v]
 
 
-@pytest.mark.parametrize("name", ["cell_sorting", "dpd_flux", "bonded_rk4", "walled", "couette", "morse_lj_lists"])
+@pytest.mark.parametrize("name", ["cell_sorting", "dpd_flux", "bonded_rk4", "walled", "couette", "morse_lj_lists",
+                                  "sums"])
 def test_round_trip_is_exact(tf, W, name):
     scene = scenes(W, tf)[name]
     eng, pots = W.build_engine(scene)
@@ -115,6 +117,26 @@ def test_settings_survive(tf, W, tmp_path):
     assert len(d["wall_potentials"]) > 0
     back = tf.Engine.load(str(path))
     assert back.to_json() == eng.to_json()
+
+
+def test_sums_and_images_survive(tf, W):
+    """Sums keep their constituents in order, each with its own offset and
+    flags; the engine's sum rule is saved with it."""
+    scene = W.sum_mix(n=600, rule=tf.SUM_ALL)
+    eng, pots = W.build_engine(scene)
+    d = json.loads(eng.to_json())
+    assert d["sum_rule"] == tf.SUM_ALL
+    back = tf.Engine.from_json(eng.to_json())
+    assert back.sum_rule == tf.SUM_ALL
+    sums = [q for q in d["potentials"] if q.get("terms")]
+    assert sorted(len(q["terms"]) for q in sums) == [2, 2, 3, 4]
+    four = [q for q in sums if len(q["terms"]) == 4][0]
+    img = scene["pots"][3]["terms"][0]["offset"]
+    assert [_pyfloat(v) for v in four["terms"][0]["offset"]] == list(img)
+    assert [_pyfloat(v) for v in four["terms"][2]["offset"]] == [-v for v in img]
+    assert four["terms"][0]["flags"] & (1 << 15) and not four["terms"][3]["flags"] & (1 << 15)
+    eng.sum_rule = tf.SUM_FIRST_IN_RANGE
+    assert json.loads(eng.to_json())["sum_rule"] == tf.SUM_FIRST_IN_RANGE
 
 
 def test_geometry_floor_quirk_kept(tf):

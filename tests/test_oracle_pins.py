@@ -289,3 +289,143 @@ def test_shifted_dpd_two_body(tf, W, O):
         fm = (10.0 + 1.0 / math.sqrt(dt)) * om
         # pi.f += f e with e = (xi - xj) / r: particle 0 is pushed to -x
         assert abs(f[0, 0] + fm) <= 1e-12 * fm and abs(f[1, 0] - fm) <= 1e-12 * fm, (r, f, fm)
+
+
+# ---- sums of potentials and periodic images (potential_eval_super_ex) ----
+
+def morse(r, d, a, r0):
+    """Potential::morse unshifted: d (e^{2a(r0-r)} - 2 e^{a(r0-r)}) and dE/dr."""
+    u = np.exp(a * (r0 - r))
+    return d * (u * u - 2 * u), d * (-2 * a * u * u + 2 * a * u)
+
+
+def _morse_spec(d, a, r0, mx, offset=None):
+    s = dict(kind="morse", d=d, a=a, r0=r0, min=0.0001, max=mx, tol=1e-8, shifted=False)
+    if offset is not None:
+        s["offset"] = offset
+    return s
+
+
+def _two_body(W, O, spec, d, rule, ids_swapped=False, dt=0.002, types=None):
+    """Particles 0 and 1 (ids in that order, or swapped in position) with
+    particle 1 at x0 + d; returns the oracle's forces on ids 0 and 1 and the
+    energy."""
+    types = types or [dict(name="A", mass=1.0, radius=0.5, dynamics=0)]
+    x0 = np.array([5.0, 5.0, 5.0])
+    x = [x0, x0 + d] if not ids_swapped else [x0 + d, x0]
+    sc = _scene(W, x, [0, 0], types, [spec], [(0, 0, 0)], L=12.0, cells=3, cutoff=3.9, dt=dt)
+    sc["sum_rule"] = rule
+    pot = W.make_potential(spec)
+    orc = O.Oracle(sc, [pot])
+    orc.force(1)
+    f = orc.particles()["f"][:, :3]
+    return f[0], f[1], orc.energies()["epot"]
+
+
+def test_sum_rules_two_body(tf, W, O):
+    """A sum of two Morse tables whose ranges overlap on [0, 1.5]: under the
+    CPU rule (pca || pcb, tf_potential_eval.h:211-215) only the first
+    constituent in range acts, under the GPU rule (every packed constituent,
+    tfRunner_cuda.cu:1432-1474) both do; past the first one's range both
+    rules give the second alone, past both nothing."""
+    m1, m2 = (1.0, 3.0, 1.0), (0.5, 2.0, 1.5)
+    spec = dict(kind="sum", terms=[_morse_spec(*m1, 1.5), _morse_spec(*m2, 2.8)])
+    u = np.array([1.0, 2.0, 2.0]) / 3.0
+    for r in (0.9, 1.3, 2.0, 3.2):
+        for rule in (tf.SUM_FIRST_IN_RANGE, tf.SUM_ALL):
+            f0, f1, ep = _two_body(W, O, spec, r * u, rule)
+            e1, de1 = morse(r, *m1)
+            e2, de2 = morse(r, *m2)
+            if r <= 1.5:
+                e, de = (e1, de1) if rule == tf.SUM_FIRST_IN_RANGE else (e1 + e2, de1 + de2)
+            elif r <= 2.8:
+                e, de = e2, de2
+            else:
+                e, de = 0.0, 0.0
+            # F on particle 0 = -(dE/dr / r) (x0 - x1) = dE/dr u
+            tol = 1e-6 * max(abs(de), 1.0)
+            assert np.abs(f0 - de * u).max() <= tol, (r, rule, f0, de)
+            assert np.abs(f1 + de * u).max() <= tol
+            assert abs(ep - e) <= 1e-6 * max(abs(e), 1.0), (r, rule, ep, e)
+
+
+def test_dpd_plus_table_sum(tf, W, O):
+    """DPD and a table on one type pair: at rest the DPD term is
+    (alpha + sigma / sqrt(dt)) omega along the separation and adds no
+    energy.  Inside both ranges the CPU rule keeps DPD (first), the GPU rule
+    adds the table."""
+    dt = 0.01
+    dpd = dict(kind="dpd", alpha=10.0, gamma=1.0, sigma=1.0, cutoff=1.0, shifted=False)
+    m = (0.3, 2.0, 1.2)
+    spec = dict(kind="sum", terms=[dpd, _morse_spec(*m, 2.4)])
+    u = np.array([2.0, -1.0, 2.0]) / 3.0
+    for r in (0.6, 1.5):
+        for rule in (tf.SUM_FIRST_IN_RANGE, tf.SUM_ALL):
+            f0, f1, ep = _two_body(W, O, spec, r * u, rule, dt=dt)
+            e_t, de_t = morse(r, *m)
+            # DPD: pi.f += f e with e = (x0 - x1) / r = -u
+            f_dpd = -(10.0 + 1.0 / math.sqrt(dt)) * (1.0 - r / 1.0) * u if r <= 1.0 else np.zeros(3)
+            f_tab = de_t * u
+            if r <= 1.0:
+                want = f_dpd if rule == tf.SUM_FIRST_IN_RANGE else f_dpd + f_tab
+                e = 0.0 if rule == tf.SUM_FIRST_IN_RANGE else e_t
+            else:
+                want, e = f_tab, e_t
+            assert np.abs(f0 - want).max() <= 1e-6 * max(np.abs(want).max(), 1.0), (r, rule, f0, want)
+            assert np.abs(f0 + f1).max() <= 1e-12 * max(np.abs(want).max(), 1.0)
+            assert abs(ep - e) <= 1e-6 * max(abs(e), 1.0)
+
+
+def _image_force(w, m):
+    """Force on the lower id when the pair is evaluated at the separation w
+    (from the lower id to the other, less the offset): -(dE/dr / |w|) w."""
+    r = float(np.linalg.norm(w))
+    e, de = morse(r, *m)
+    return -de / r * w, e
+
+
+def test_periodic_offset_two_body(tf, W, O):
+    """A lone periodic image (Potential::offset, POTENTIAL_PERIODIC): the
+    pair is evaluated at dx - offset (tf_potential_eval.h:220-225), dx from
+    the lower id to the other, whichever particle sits where.  A mirrored
+    pair of images summed under the GPU rule does not depend on that
+    orientation."""
+    m = (0.5, 3.0, 0.8)
+    off = np.array([0.35, -0.2, 0.1])
+    d = np.array([0.9, 0.4, -0.3])   # x1 - x0 when id 0 is at x0
+    lone = _morse_spec(*m, 1.8, offset=tuple(off))
+    for swapped in (False, True):
+        f0, f1, ep = _two_body(W, O, lone, d, tf.SUM_ALL, ids_swapped=swapped)
+        dx = -d if not swapped else d        # x(id 0) - x(id 1)
+        want, e = _image_force(dx - off, m)
+        assert np.abs(f0 - want).max() <= 1e-6 * max(np.abs(want).max(), 1.0), (swapped, f0, want)
+        assert np.abs(f1 + f0).max() <= 1e-12 * max(np.abs(want).max(), 1.0)
+        assert abs(ep - e) <= 1e-6 * max(abs(e), 1.0)
+        # the other sign of the offset gives a different answer here
+        other, _ = _image_force(dx + off, m)
+        assert np.abs(other - want).max() > 1e-2 * np.abs(want).max()
+    mirrored = dict(kind="sum", terms=[_morse_spec(*m, 1.8, offset=tuple(off)),
+                                       _morse_spec(*m, 1.8, offset=tuple(-off))])
+    for swapped in (False, True):
+        f0, f1, ep = _two_body(W, O, mirrored, d, tf.SUM_ALL, ids_swapped=swapped)
+        dx = -d if not swapped else d
+        wa, ea = _image_force(dx - off, m)
+        wb, eb = _image_force(dx + off, m)
+        assert np.abs(f0 - (wa + wb)).max() <= 1e-6 * max(np.abs(wa + wb).max(), 1.0)
+        assert abs(ep - (ea + eb)) <= 1e-6 * max(abs(ea + eb), 1.0)
+
+
+def test_sum_mix_rules_differ_and_conserve_momentum(tf, W, O):
+    """The sum_mix scene under both rules: pair forces cancel (Newton 3,
+    the images included), and the rules give different forces."""
+    out = []
+    for rule in (tf.SUM_FIRST_IN_RANGE, tf.SUM_ALL):
+        sc = W.sum_mix(n=1500, rule=rule)
+        pots = [W.make_potential(p) for p in sc["pots"]]
+        orc = O.Oracle(sc, pots)
+        orc.force(2)
+        f = orc.particles()["f"][:, :3]
+        scale = math.sqrt(float(np.mean(np.sum(f ** 2, axis=1))))
+        assert np.abs(f.sum(axis=0)).max() <= 1e-9 * scale * f.shape[0]
+        out.append(f)
+    assert np.abs(out[0] - out[1]).max() > 0.1 * math.sqrt(float(np.mean(np.sum(out[1] ** 2, axis=1))))

@@ -29,6 +29,8 @@ PREC_MIXED = 0
 PREC_FP64 = 1
 FORWARD_EULER = 0
 RUNGE_KUTTA_4 = 1
+SUM_FIRST_IN_RANGE = 0
+SUM_ALL = 1
 # boundary condition kinds (bit sets) and faces
 BOUNDARY_VELOCITY = 1
 BOUNDARY_PERIODIC = 2
@@ -73,6 +75,8 @@ _SIGNATURES = {
     "tfg_engine_set_nspecies": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tfg_engine_set_integrator": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tfg_engine_integrator": (ctypes.c_int, [_vp]),
+    "tfg_engine_set_sum_rule": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfg_engine_sum_rule": (ctypes.c_int, [_vp]),
     "tfg_engine_set_boundary": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint32, _dp, ctypes.c_double,
                                                ctypes.c_double]),
     "tfg_engine_set_wall_potential": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
@@ -102,6 +106,11 @@ _SIGNATURES = {
     "tfg_potential_info": (ctypes.c_int, [_vp, _ip, _u32p, _ip, _dp, _dp, _dp, _dp]),
     "tfg_potential_coeffs": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
     "tfg_potential_eval": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]),
+    "tfg_potential_sum": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _vp]),
+    "tfg_potential_set_offset": (ctypes.c_int, [_vp, _dp]),
+    "tfg_potential_offset": (ctypes.c_int, [_vp, _dp]),
+    "tfg_potential_terms": (ctypes.c_int, [_vp]),
+    "tfg_potential_term": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
     "tfg_engine_addpot": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int]),
     "tfg_engine_addflux": (ctypes.c_int, [_vp] + [ctypes.c_int] * 5 + [ctypes.c_double] * 4),
     "tfg_engine_addparts": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _i32p]),
@@ -339,6 +348,36 @@ class Potential:
         lib().tfg_potential_coeffs(self._h, _dptr(out), cnt)
         return out
 
+    def __add__(self, other):
+        """The sum of two potentials (Potential::operator+), flattened into
+        at most 7 constituents."""
+        if not isinstance(other, Potential):
+            return NotImplemented
+        return Potential._make(lib().tfg_potential_sum, self._h, other._h)
+
+    @property
+    def offset(self):
+        """Periodic image offset (Potential::offset); setting it marks the
+        potential periodic, so the pair is evaluated at dx - offset."""
+        out = np.zeros(3, dtype=np.float64)
+        _check(lib().tfg_potential_offset(self._h, _dptr(out)))
+        return out
+
+    @offset.setter
+    def offset(self, value):
+        v = _f64(value, 3)
+        _check(lib().tfg_potential_set_offset(self._h, _dptr(v)))
+
+    def constituents(self):
+        """Copies of the constituents of a sum (Potential::constituents);
+        empty for a single potential."""
+        out = []
+        for k in range(_check(lib().tfg_potential_terms(self._h))):
+            h = _vp()
+            _check(lib().tfg_potential_term(self._h, k, ctypes.byref(h)))
+            out.append(Potential(h))
+        return out
+
     def __call__(self, r, ri=0.0, rj=0.0):
         """Energy and force/r at distance r (potential_eval_ex semantics)."""
         e = ctypes.c_double()
@@ -406,6 +445,17 @@ class Engine:
     @integrator.setter
     def integrator(self, kind):
         _check(lib().tfg_engine_set_integrator(self._h, int(kind)))
+
+    @property
+    def sum_rule(self):
+        """SUM_FIRST_IN_RANGE (the CPU engine: the first constituent of a sum
+        whose range holds the pair) or SUM_ALL (the GPU path: every
+        constituent in range)."""
+        return _check(lib().tfg_engine_sum_rule(self._h))
+
+    @sum_rule.setter
+    def sum_rule(self, rule):
+        _check(lib().tfg_engine_set_sum_rule(self._h, int(rule)))
 
     def set_boundary(self, face, kind, velocity=None, restore=1.0, radius=0.0):
         """One face's boundary condition (face index or name: left, right,

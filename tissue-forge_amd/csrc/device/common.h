@@ -35,7 +35,9 @@ struct SysDesc {
     int ncells, cell_off, p_off, np, ntypes;
     int nstencil, stencil_off;   // columns (ox, oy) of the neighbour stencil
     int span_z;                  // z extent of every column: -span_z .. span_z
-    int pot_off, type_off;
+    int pot_off, nslots;          // descriptor slots (int2: potential, next slot) from pot_off
+    int type_off;
+    int sum_all;                  // a sum evaluates every constituent (else up to the first in range)
     int flux_off;     // first (type pair) entry of this system in the flux map
     int flux_first;   // first flux entry of this system; the map's offsets are relative to it
     int nflux;
@@ -90,7 +92,10 @@ __host__ __device__ inline size_t list_index(int k, int i, int lcap) {
 // vectors (one per type pair, ti * ntypes + tj), so that lanes reading the
 // descriptors of different type pairs spread over the LDS banks.  A particle
 // carries its type as the byte offset of its column, type * sizeof(v4).
-constexpr int kPairVecs = 3;
+// Slots 0 .. ntypes^2 - 1 hold each type pair's potential (the first
+// constituent of a sum); the further constituents of sums take the slots
+// after them, chained in order through their descriptors.
+constexpr int kPairVecs = 4;
 constexpr int kDescPairs = kMaxTypes * kMaxTypes;
 template <typename v4> __host__ __device__ constexpr int type_column(int type) { return type * static_cast<int>(sizeof(v4)); }
 
@@ -100,7 +105,9 @@ template <typename real> struct PotDesc {
     int n, coff;
     real a, b, r0p1, al0, al1, al2;
     real dalpha, dgamma, dsigma;
+    real offx, offy, offz;   // periodic image offset (POTENTIAL_PERIODIC)
     double bd;   // b in FP64 (the FP32 path's cutoff guard)
+    double offd[3];          // the offset in FP64 (the guard)
 };
 
 struct TypeDesc {
@@ -149,7 +156,7 @@ template <typename real> struct KP {
     const int *cell_sys;
     const int *stencil;   // 2 ints (ox, oy) per stencil column
     const PotDesc<real> *pots;
-    const int *potmap;
+    const int2 *slots;    // per system: descriptor slot -> (potential, next slot or 0)
     const real *coef;
     const TypeDesc *types;
     const FluxDesc<real> *flux;

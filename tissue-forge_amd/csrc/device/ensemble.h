@@ -50,7 +50,9 @@ struct DeviceEnsemble {
     std::vector<SysDesc> sys;
     int nsys = 0, np = 0, ncells = 0, S = 0;
     int max_sys_blocks = 1;   // energy partial blocks per system
-    bool has_dpd = false, has_flux = false, has_radius = false;
+    // has_ext: some pair potential needs the extended dispatch (radius
+    // transforms, sums, periodic images)
+    bool has_dpd = false, has_flux = false, has_ext = false;
     bool lean_advance = false;   // no bonds, angles, active walls or species: the lean update kernel
     int integrator = INTEG_FORWARD_EULER;   // shared by every system of the ensemble
     int nr_fluxsteps = 1;                   // shared as well
@@ -80,7 +82,8 @@ struct DeviceEnsemble {
     int prof_steps = 0;
 
     SysDesc *d_sys = nullptr;
-    int *d_cell_sys = nullptr, *d_stencil = nullptr, *d_potmap = nullptr, *d_fluxmap = nullptr;
+    int *d_cell_sys = nullptr, *d_stencil = nullptr, *d_fluxmap = nullptr;
+    int2 *d_slots = nullptr;
     int *d_wallmap = nullptr;
     void *d_pots = nullptr, *d_coef = nullptr, *d_flux = nullptr;
     TypeDesc *d_types = nullptr;

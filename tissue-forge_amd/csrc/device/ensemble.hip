@@ -24,7 +24,7 @@ template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy) {
     P.cell_sys = E->d_cell_sys;
     P.stencil = E->d_stencil;
     P.pots = static_cast<const PotDesc<real> *>(E->d_pots);
-    P.potmap = E->d_potmap;
+    P.slots = E->d_slots;
     P.coef = static_cast<const real *>(E->d_coef);
     P.types = E->d_types;
     P.flux = static_cast<const FluxDesc<real> *>(E->d_flux);
@@ -259,7 +259,8 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     const bool centred = std::is_same<real, float>::value;
 
     std::vector<SysDesc> sys(nsys);
-    std::vector<int> cell_sys, stencil, potmap, fluxmap, wallmap;
+    std::vector<int> cell_sys, stencil, fluxmap, wallmap;
+    std::vector<int2> slots;
     std::vector<PotDesc<real>> pots;
     std::vector<real> coef;
     std::vector<TypeDesc> types;
@@ -360,9 +361,9 @@ template <typename real> static int upload(DeviceEnsemble *E) {
         d.time0 = e.time;
         std::vector<int> &remap = remaps[s];
         remap.assign(e.pots.size(), -1);
-        auto add_pot = [&](int pid) -> int {
-            if (remap[pid] >= 0) return remap[pid];
-            const Potential &p = *e.pots[pid];
+        // A device descriptor of one table or DPD record (a sum's constituent
+        // or a whole non-sum potential).
+        auto add_desc = [&](const Potential &p) -> int {
             PotDesc<real> pd{};
             pd.kind = static_cast<int>(p.kind);
             pd.flags = p.flags;
@@ -378,25 +379,65 @@ template <typename real> static int upload(DeviceEnsemble *E) {
             pd.dalpha = real(p.dpd_alpha);
             pd.dgamma = real(p.dpd_gamma);
             pd.dsigma = real(p.dpd_sigma);
+            pd.offx = real(p.offset[0]);
+            pd.offy = real(p.offset[1]);
+            pd.offz = real(p.offset[2]);
+            for (int k = 0; k < 3; k++) pd.offd[k] = p.offset[k];
             for (double cval : p.c) coef.push_back(real(cval));
-            remap[pid] = static_cast<int>(pots.size());
             pots.push_back(pd);
+            return static_cast<int>(pots.size()) - 1;
+        };
+        auto add_pot = [&](int pid) -> int {   // bonds, angles, walls: one record
+            if (remap[pid] < 0) remap[pid] = add_desc(*e.pots[pid]);
             return remap[pid];
         };
-        d.pot_off = static_cast<int>(potmap.size());
+        // Descriptor slots: slot t < nt^2 holds type pair t's potential (the
+        // first constituent of a sum), the further constituents of a sum take
+        // slots after them, linked in order (0 ends a chain).
+        std::vector<std::vector<int>> sum_descs(e.pots.size());
+        std::vector<int2> sys_slots(static_cast<size_t>(nt) * nt, make_int2(-1, 0));
+        auto note_flags = [&](const Potential &p) {
+            if (p.kind == KIND_DPD) E->has_dpd = true;
+            if (p.flags & (POT_SHIFTED | POT_SCALED | POT_PERIODIC)) E->has_ext = true;
+            if (p.b < e.geom.cutoff * (1.0 - 4.0 * kGuard) || (p.flags & (POT_SHIFTED | POT_SCALED | POT_PERIODIC)))
+                d.binside = 1;
+        };
         d.coef_begin = static_cast<int>(coef.size());
         for (int a = 0; a < nt; a++)
             for (int b = 0; b < nt; b++) {
                 const int pid = e.potmap[a * e.max_types + b];
-                if (pid >= 0) {
-                    const Potential &p = *e.pots[pid];
-                    if (p.kind == KIND_DPD) E->has_dpd = true;
-                    if (p.flags & (POT_SHIFTED | POT_SCALED)) E->has_radius = true;
-                    if (p.b < e.geom.cutoff * (1.0 - 4.0 * kGuard) || (p.flags & (POT_SHIFTED | POT_SCALED)))
-                        d.binside = 1;
+                if (pid < 0) continue;
+                const Potential &p = *e.pots[pid];
+                const int t = a * nt + b;
+                if (p.kind != KIND_SUM) {
+                    note_flags(p);
+                    sys_slots[t] = make_int2(add_pot(pid), 0);
+                    continue;
                 }
-                potmap.push_back(pid >= 0 ? add_pot(pid) : -1);
+                E->has_ext = true;
+                d.binside = 1;   // the guard looks at every constituent's range end
+                if (sum_descs[pid].empty())
+                    for (const Potential &term : p.terms) {
+                        note_flags(term);
+                        sum_descs[pid].push_back(add_desc(term));
+                    }
+                const std::vector<int> &ds = sum_descs[pid];
+                sys_slots[t] = make_int2(ds[0], 0);
+                int prev = t;
+                for (size_t k = 1; k < ds.size(); k++) {
+                    const int slot = static_cast<int>(sys_slots.size());
+                    sys_slots[prev].y = slot;
+                    sys_slots.push_back(make_int2(ds[k], 0));
+                    prev = slot;
+                }
             }
+        if (static_cast<int>(sys_slots.size()) > kDescPairs)
+            return set_error(TFG_E_RANGE, "the type pairs and sum constituents of a system need more than " +
+                                              std::to_string(kDescPairs) + " descriptor slots");
+        d.pot_off = static_cast<int>(slots.size());
+        d.nslots = static_cast<int>(sys_slots.size());
+        d.sum_all = e.sum_rule == SUM_ALL ? 1 : 0;
+        slots.insert(slots.end(), sys_slots.begin(), sys_slots.end());
         d.coef_count = static_cast<int>(coef.size()) - d.coef_begin;
         max_coef = std::max(max_coef, d.coef_count);
         d.type_off = static_cast<int>(types.size());
@@ -459,7 +500,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     for (const SysDesc &d : sys) max_nt = std::max(max_nt, d.ntypes);
     // Lists carry tables only (no DPD, flux or radius transforms), with every
     // system's tables in LDS, and slots below 2^25.
-    E->list_mode = list_ok && !E->has_dpd && !E->has_flux && !E->has_radius && np + 1 < (1 << kListSlotBits) &&
+    E->list_mode = list_ok && !E->has_dpd && !E->has_flux && !E->has_ext && np + 1 < (1 << kListSlotBits) &&
                    static_cast<size_t>(max_coef) * sizeof(real) <= 16384 && max_coef > 0;
     E->lcap = E->list_mode ? lcap : 0;
     int rc = nonbond_setup<real>(E, max_nt, max_coef);
@@ -547,7 +588,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     ALLOC(E->d_sys, SysDesc, nsys);
     ALLOC(E->d_cell_sys, int, nc);
     ALLOC(E->d_stencil, int, stencil.size());
-    ALLOC(E->d_potmap, int, potmap.size());
+    ALLOC(E->d_slots, int2, slots.size());
     ALLOC(E->d_fluxmap, int, fluxmap.size());
     ALLOC(E->d_wallmap, int, wallmap.size());
     ALLOC(E->d_pots, PotDesc<real>, pots.size());
@@ -608,7 +649,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     if ((rc = H2D(E->d_sys, sys.data(), sizeof(SysDesc) * nsys))) return rc;
     if ((rc = H2D(E->d_cell_sys, cell_sys.data(), sizeof(int) * cell_sys.size()))) return rc;
     if ((rc = H2D(E->d_stencil, stencil.data(), sizeof(int) * stencil.size()))) return rc;
-    if ((rc = H2D(E->d_potmap, potmap.data(), sizeof(int) * potmap.size()))) return rc;
+    if ((rc = H2D(E->d_slots, slots.data(), sizeof(int2) * slots.size()))) return rc;
     if ((rc = H2D(E->d_fluxmap, fluxmap.data(), sizeof(int) * fluxmap.size()))) return rc;
     if ((rc = H2D(E->d_wallmap, wallmap.data(), sizeof(int) * wallmap.size()))) return rc;
     if ((rc = H2D(E->d_pots, pots.data(), sizeof(PotDesc<real>) * pots.size()))) return rc;

@@ -51,7 +51,9 @@ constexpr int kListShiftZ = 2048;
 // different type pairs on different banks):
 //   table: q0 = (a, b, alpha0, alpha1), q1 = (alpha2, n, table bits, meta bits)
 //   DPD:   q0 = (a, b, alpha, gamma),    q1 = (sigma, 1/b, 0, meta bits)
-//   both:  q2 = (r0+1, b_hi, b_lo, 0) with b = b_hi + b_lo to 48 bits (the FP64 guard)
+//   both:  q2 = (r0+1, b_hi, b_lo, next slot) with b = b_hi + b_lo to 48 bits (the
+//          FP64 guard) and the slot of a sum's next constituent (0: none),
+//          q3 = (periodic image offset, 0)
 // An absent pair has a = 0, b = -1, so that no distance is in range.  The
 // table bits are the LDS byte offset of the table's first interval (from the
 // start of the dynamic LDS, in the first-half array) when the tables are
@@ -64,30 +66,30 @@ constexpr int kMetaDpd = 1 << 16;
 constexpr int kMetaShifted = 1 << 20;
 constexpr int kMetaScaled = 1 << 21;
 constexpr int kMetaBInside = 1 << 22;
+constexpr int kMetaPeriodic = 1 << 23;
 
-// FP64 squared distance of a guarded pair: pu is the partner's stored
-// (unshifted) position, pj its staged copy in the i frame.  The shift between
-// them is a whole number of cells (the staged copy is within 1e-6 of it), so
-// it is found in FP32; each axis then adds (pi - pu - shift)^2 in FP64.
-__device__ inline double guard_r2(const float4 *pu_ptr, float4 pi, float4 pj, const double *h, const float *ih) {
+// FP64 separation of a guarded pair: pu is the partner's stored (unshifted)
+// position, pj its staged copy in the i frame.  The shift between them is a
+// whole number of cells (the staged copy is within 1e-6 of it), so it is
+// found in FP32; each axis is then pi - pu - shift in FP64.
+__device__ inline void guard_sep(const float4 *pu_ptr, float4 pi, float4 pj, const double *h, const float *ih,
+                                 double d[3]) {
     const float4 pu = *pu_ptr;
     const float pa[3] = {pi.x, pi.y, pi.z}, ja[3] = {pj.x, pj.y, pj.z}, ua[3] = {pu.x, pu.y, pu.z};
-    double r2 = 0.0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const float cells = rintf((ja[a] - ua[a]) * ih[a]);
-        const double e = (static_cast<double>(pa[a]) - static_cast<double>(ua[a])) - static_cast<double>(cells) * h[a];
-        r2 = __builtin_fma(e, e, r2);
+        d[a] = (static_cast<double>(pa[a]) - static_cast<double>(ua[a])) - static_cast<double>(cells) * h[a];
     }
-    return r2;
 }
-__device__ inline double guard_r2(const double4 *, double4, double4, const double *, const float *) { return 0.0; }
+__device__ inline void guard_sep(const double4 *, double4, double4, const double *, const float *, double d[3]) {
+    d[0] = d[1] = d[2] = 0.0;
+}
 
 // Evaluation modes of a pair (the FP32 cutoff guard).
-constexpr int kFree = 0, kDecidedIn = 1, kDecidedOut = 2;
+constexpr int kFree = 0, kDecided = 1;
 constexpr std::integral_constant<int, kFree> kFreeMode{};
-constexpr std::integral_constant<int, kDecidedIn> kDecidedInMode{};
-constexpr std::integral_constant<int, kDecidedOut> kDecidedOutMode{};
+constexpr std::integral_constant<int, kDecided> kDecidedMode{};
 
 // Squared distance with a fixed rounding (explicit FMAs, no contraction), so
 // that the pair loop and the guard's rescan classify every pair identically.
@@ -101,6 +103,9 @@ __device__ inline double dist2(double4 a, double4 b) {
     const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
     return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
 }
+
+__device__ inline float fma_of(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_of(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -116,7 +121,9 @@ template <typename real>
 NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, bool radius, int max_chunk,
                    bool build = false) {
     using v4 = typename V<real>::v4;
-    const bool ids = dpd || flux;
+    // ids: DPD noise and flux roles need the partner's id, periodic images
+    // orient the pair by the lower id
+    const bool ids = dpd || flux || radius;
     // the cutoff guard's lookups; the list build's entries
     const bool slots = std::is_same<real, float>::value || build;
     NbLayout L{};
@@ -150,13 +157,15 @@ NbLayout nb_layout(int max_nt, int coef_lds_count, int S, bool dpd, bool flux, b
     return L;
 }
 
-// Pair descriptor of type pair t of system sd (the layout above); off_coef
-// is the LDS byte offset of the system's tables when they are cached.
+// Descriptor of slot t of system sd (the layout above; slots below ntypes^2
+// are the type pairs); off_coef is the LDS byte offset of the system's tables
+// when they are cached.
 template <typename real, bool CLDS>
 __device__ __forceinline__ void write_pair_descriptor(const KP<real> &P, const SysDesc &sd, int t, int off_coef,
                                                       typename V<real>::v4 *s_pair) {
     using v4 = typename V<real>::v4;
-    const int pi = P.potmap[sd.pot_off + t];
+    const int2 slot = P.slots[sd.pot_off + t];
+    const int pi = slot.x;
     v4 q0, q1, q2, q3;
     q0.x = q0.y = q0.z = q0.w = real(0);
     q1 = q2 = q3 = q0;
@@ -165,8 +174,13 @@ __device__ __forceinline__ void write_pair_descriptor(const KP<real> &P, const S
     if (pi >= 0) {
         const PotDesc<real> pd = P.pots[pi];
         meta = (pd.kind == KIND_DPD ? kMetaDpd : 0) | ((pd.flags & POT_SHIFTED) ? kMetaShifted : 0) |
-               ((pd.flags & POT_SCALED) ? kMetaScaled : 0) |
-               (pd.bd < sd.cutoff * (1.0 - 4.0 * kGuard) || (pd.flags & (POT_SHIFTED | POT_SCALED)) ? kMetaBInside : 0);
+               ((pd.flags & POT_SCALED) ? kMetaScaled : 0) | ((pd.flags & POT_PERIODIC) ? kMetaPeriodic : 0) |
+               (pd.bd < sd.cutoff * (1.0 - 4.0 * kGuard) || (pd.flags & (POT_SHIFTED | POT_SCALED | POT_PERIODIC))
+                    ? kMetaBInside
+                    : 0);
+        q3.x = pd.offx;
+        q3.y = pd.offy;
+        q3.z = pd.offz;
         q0.x = pd.a;
         q0.y = pd.b;
         q2.y = real(pd.bd);
@@ -188,10 +202,11 @@ __device__ __forceinline__ void write_pair_descriptor(const KP<real> &P, const S
         q2.x = pd.r0p1;
     }
     q1.w = as_real_bits(meta, real(0));
-    (void)q3;
+    q2.w = as_real_bits(slot.y, real(0));   // the next constituent's slot, 0 at the end
     s_pair[t] = q0;
     s_pair[kDescPairs + t] = q1;
     s_pair[2 * kDescPairs + t] = q2;
+    s_pair[3 * kDescPairs + t] = q3;
 }
 
 // Copies a system's table coefficients into LDS split by halves: the first
@@ -214,10 +229,11 @@ constexpr int kStageBatch = 4;   // list build: staging passes whose loads are i
 // test then marks the pairs, and the set bits are written to the lists
 // (partner slot and build-cell offset, in candidate order) instead of being
 // evaluated.  It runs only in the steps that rebuild.
-template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY, bool BUILD = false>
+template <typename real, bool DPD, bool FLUX, bool EXT, bool CLDS, bool ENERGY, bool BUILD = false>
 __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
     using v4 = typename V<real>::v4;
     constexpr bool kIds = DPD || FLUX;   // pair terms that need the partner's id
+    constexpr bool kStageIds = kIds || EXT;   // periodic images also compare ids
     constexpr bool kGuarded = std::is_same<real, float>::value && !BUILD;
     if (BUILD && P.lflags[0] == 0) return;
     constexpr int kCol = type_column<v4>(1);   // bytes per descriptor column
@@ -272,10 +288,11 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
         const int nt = sd.ntypes;
         // Pair descriptors (and tables) of this system.
         __syncthreads();
-        for (int t = threadIdx.x; t < nt * nt; t += kNbThreads) {
+        for (int t = threadIdx.x; t < sd.nslots; t += kNbThreads)
             write_pair_descriptor<real, CLDS>(P, sd, t, P.nb.off_coef, s_pair);
-            if (FLUX) s_fmap[t] = make_int2(P.fluxmap[2 * (sd.flux_off + t)], P.fluxmap[2 * (sd.flux_off + t) + 1]);
-        }
+        if (FLUX)
+            for (int t = threadIdx.x; t < nt * nt; t += kNbThreads)
+                s_fmap[t] = make_int2(P.fluxmap[2 * (sd.flux_off + t)], P.fluxmap[2 * (sd.flux_off + t) + 1]);
         if (FLUX)
             for (int t = threadIdx.x; t < sd.nflux; t += kNbThreads) s_flux[t] = P.flux[sd.flux_first + t];
         if (CLDS) cache_tables(P, sd, s_dyn + P.nb.off_coef, P.nb.coef_half);
@@ -397,86 +414,66 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                 int lbase = 0;     // BUILD: list entries of this i particle so far
                 if (BUILD) s_self[lane] = -1;
 
-                // One candidate k of the staged list; ok = false makes it a no-op
-                // (k is then 0, a staged entry, so every read stays in range).
-                // Modes of the FP32 cutoff guard: kFree defers a pair inside the
-                // band around the cutoff (or, for a table that ends below the
-                // cutoff, around its end b) to the rescan after the round; the
-                // rescan evaluates it with kDecidedIn / kDecidedOut once FP64 has
-                // placed it inside the cutoff, and inside or past b (past b: the
-                // number density only).
-                bool deferred = false;
-                auto eval_pair = [&](int k, bool ok, auto mode) {
-                    constexpr int kMode = decltype(mode)::value;
-                    const v4 pj = s_pos[k];
-                    const real dx = pi4.x - pj.x;
-                    const real dy = pi4.y - pj.y;
-                    const real dz = pi4.z - pj.z;
-                    const real r2 = dist2(pi4, pj);
-                    // below eps the inverse distance is 1/eps (r itself only
-                    // enters where it is clamped to the table's a, or through
-                    // W(r) ~ W(0))
-                    const real ir = rsq(fmax(r2, eps2));
-                    const real r_raw = r2 * ir;
-                    const int tpb = ti_row + real_bits_as_int(pj.w);   // descriptor byte offset
-                    const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + tpb);
-                    const v4 q0 = qd[0];
-                    const v4 q1 = qd[kDescPairs];
-                    const int meta = real_bits_as_int(q1.w);
-                    // distance after the radius transform of the pair's potential
-                    real rt = r_raw;
-                    if (RADIUS) {
-                        if ((meta & kMetaScaled) && !(meta & kMetaDpd))
-                            rt = r_raw / (pi4.w + s_rad[k]);
-                        else if (meta & kMetaShifted)
-                            rt = r_raw - (pi4.w + s_rad[k]) + ((meta & kMetaDpd) ? real(0) : qd[2 * kDescPairs].x);
+                // Distance of a constituent after its radius transform
+                // (potential_eval_adjust_distance, tf_potential_eval.h:43-51;
+                // DPD shifts only, tf_dpd_eval.h:41).
+                auto transformed = [&](real r, int mt, real r0p1, int k) -> real {
+                    if (!EXT) return r;
+                    if ((mt & kMetaScaled) && !(mt & kMetaDpd)) return r / (pi4.w + s_rad[k]);
+                    if (mt & kMetaShifted) return r - (pi4.w + s_rad[k]) + ((mt & kMetaDpd) ? real(0) : r0p1);
+                    return r;
+                };
+                // Separation from the periodic image a constituent refers to
+                // (potential_eval_super_ex, tf_potential_eval.h:220-225): dx -
+                // offset with dx taken from the lower id to the higher, so
+                // dx + offset from the higher id's side.  (The reference
+                // orients the pair by its traversal; a lone image, or two
+                // mirrored images both in range under the first-in-range
+                // rule, then depends on that order.  The lower-id orientation
+                // fixes it and agrees everywhere else.)
+                auto image = [&](const v4 *qt, int mt, real &tdx, real &tdy, real &tdz, real &tr2, real &tir,
+                                 real &traw, int k) {
+                    if (!EXT || !(mt & kMetaPeriodic)) return;
+                    const v4 off = qt[3 * kDescPairs];
+                    const real sg = mi.y < s_id[k] ? real(1) : real(-1);
+                    tdx -= sg * off.x;
+                    tdy -= sg * off.y;
+                    tdz -= sg * off.z;
+                    tr2 = fma_of(tdz, tdz, fma_of(tdy, tdy, tdx * tdx));
+                    tir = rsq(fmax(tr2, eps2));
+                    traw = tr2 * tir;
+                };
+                // The FP32 cutoff guard's band: near the cutoff, or near the
+                // end b (in its own transformed distance) of a constituent
+                // whose range ends inside the cutoff.
+                auto in_band = [&](const v4 *qd, real dx, real dy, real dz, real r2, real ir, real r_raw, int k) {
+                    bool band = r2 > cut2_lo;
+                    if (!binside) return band;
+                    const v4 *qt = qd;
+                    for (int t = 0; t < kMaxTerms; t++) {
+                        const v4 a0 = qt[0], a1 = qt[kDescPairs], a2 = qt[2 * kDescPairs];
+                        const int mt = real_bits_as_int(a1.w);
+                        real tdx = dx, tdy = dy, tdz = dz, tr2 = r2, tir = ir, traw = r_raw;
+                        image(qt, mt, tdx, tdy, tdz, tr2, tir, traw, k);
+                        const real rt = transformed(traw, mt, a2.x, k);
+                        band = band || ((mt & kMetaBInside) && fabs(rt - a0.y) <= real(2.0 * kGuard) * a0.y);
+                        const int next = EXT ? real_bits_as_int(a2.w) : 0;
+                        if (next == 0) break;
+                        qt = reinterpret_cast<const v4 *>(s_dyn + next * kCol);
                     }
-                    if (kGuarded && kMode == kFree) {
-                        bool band = r2 > cut2_lo;
-                        if (binside) band = band || ((meta & kMetaBInside) && fabs(rt - q0.y) <= real(2.0 * kGuard) * q0.y);
-                        band = band && ok;
-                        deferred = deferred || band;
-                        ok = ok && !band;
-                    }
-                    nd += ok ? w0 + r2 * (w2 + r_raw * w3) : real(0);
-                    int id_j = 0;
-                    if (kIds) id_j = s_id[k] - sd.p_off;
-                    if (FLUX && ok) {
-                        const int tj = real_bits_as_int(pj.w) / kCol;
-                        const int2 fm = s_fmap[ti * nt + tj];
-                        for (int f = 0; f < fm.y; f++) {
-                            const FluxDesc<real> &fd = s_flux[fm.x + f];
-                            if (r_raw > fd.cutoff) continue;
-                            real term = real(1) - r_raw / fd.cutoff;
-                            term = term * term;
-                            // Role of this particle in the entry (tf_flux_eval.h:64-67):
-                            // the type_a side; between equal types the lower id.
-                            const bool i_is_a = (ti == fd.type_a) && (ti != tj || id_i < id_j);
-                            const real sa = i_is_a ? spi[fd.index_a] : s_sp[k * S + fd.index_a];
-                            const real sb = i_is_a ? s_sp[k * S + fd.index_b] : spi[fd.index_b];
-                            real qv;
-                            if (fd.kind == FLUX_FICK) {
-                                qv = fd.coef * (sa - sb);
-                            } else if (fd.kind == FLUX_SECRETE) {
-                                qv = fd.coef * (sa - fd.target);
-                                qv = qv > real(0) ? qv : real(0);
-                            } else {
-                                qv = fd.coef * (fd.target - sb) * sa;
-                                qv = qv > real(0) ? qv : real(0);
-                            }
-                            qv *= term;
-                            if (i_is_a)
-                                qacc[fd.index_a] += -qv - fd.half_decay * sa;
-                            else
-                                qacc[fd.index_b] += qv - fd.half_decay * sb;
-                        }
-                    }
-                    // inside the table's range b (the FP64 decision for guarded pairs)
-                    const bool in = kMode == kFree ? ok && rt <= q0.y : (kMode == kDecidedIn);
-                    if (DPD && (meta & kMetaDpd)) {
-                        const real ro = r2 < eps2 ? eps_of<real>() : r_raw;
+                    return band;
+                };
+                // One constituent (DPD or table) of the pair potential,
+                // applied when in (potential_eval_super_ex's dispatch).
+                auto eval_term = [&](const v4 *qt, real tdx, real tdy, real tdz, real tr2, real tir, real traw,
+                                     real rt, bool in, int id_j, int k) {
+                    const v4 q0 = qt[0];
+                    const v4 q1 = qt[kDescPairs];
+                    const int mt = real_bits_as_int(q1.w);
+                    if (DPD && (mt & kMetaDpd)) {
+                        const real ro = tr2 < eps2 ? eps_of<real>() : traw;
                         const real r = rt >= q0.x ? rt : q0.x;
-                        const real ex = dx * ir, ey = dy * ir, ez = dz * ir;
+                        const real ex = tdx * tir, ey = tdy * tir, ez = tdz * tir;
                         const v4 vj = s_vel[k];
                         const real ev = ex * (vi4.x - vj.x) + ey * (vi4.y - vj.y) + ez * (vi4.z - vj.z);
                         const real om = r < real(0) ? real(1) : (real(1) - r * q1.y);
@@ -513,13 +510,95 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     eff = eff * x + ee; ee = ee * x + c1.y;
                     eff = eff * x + ee; ee = ee * x + c1.z;
                     eff = eff * x + ee; ee = ee * x + c1.w;
-                    const real f = in ? eff * c0.y * ir : real(0);
-                    fx -= f * dx;
-                    fy -= f * dy;
-                    fz -= f * dz;
+                    const real f = in ? eff * c0.y * tir : real(0);
+                    fx -= f * tdx;
+                    fy -= f * tdy;
+                    fz -= f * tdz;
                     if (ENERGY) {
                         ep += in ? ee : real(0);
-                        vir -= f * r2;
+                        vir -= f * tr2;
+                    }
+                };
+                // One candidate k of the staged list; ok = false makes it a no-op
+                // (k is then 0, a staged entry, so every read stays in range).
+                // Modes of the FP32 cutoff guard: kFree defers a pair inside the
+                // band (in_band) to the rescan after the round; the rescan
+                // evaluates it with kDecided once FP64 has placed it inside the
+                // cutoff, with bit t of inmask telling whether constituent t is
+                // in range (none: the number density only).
+                bool deferred = false;
+                auto eval_pair = [&](int k, bool ok, auto mode, unsigned inmask) {
+                    constexpr int kMode = decltype(mode)::value;
+                    const v4 pj = s_pos[k];
+                    const real dx = pi4.x - pj.x;
+                    const real dy = pi4.y - pj.y;
+                    const real dz = pi4.z - pj.z;
+                    const real r2 = dist2(pi4, pj);
+                    // below eps the inverse distance is 1/eps (r itself only
+                    // enters where it is clamped to the table's a, or through
+                    // W(r) ~ W(0))
+                    const real ir = rsq(fmax(r2, eps2));
+                    const real r_raw = r2 * ir;
+                    const int tpb = ti_row + real_bits_as_int(pj.w);   // descriptor byte offset
+                    const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + tpb);
+                    if (kGuarded && kMode == kFree) {
+                        const bool band = ok && in_band(qd, dx, dy, dz, r2, ir, r_raw, k);
+                        deferred = deferred || band;
+                        ok = ok && !band;
+                    }
+                    nd += ok ? w0 + r2 * (w2 + r_raw * w3) : real(0);
+                    int id_j = 0;
+                    if (kIds) id_j = s_id[k] - sd.p_off;
+                    if (FLUX && ok) {
+                        const int tj = real_bits_as_int(pj.w) / kCol;
+                        const int2 fm = s_fmap[ti * nt + tj];
+                        for (int f = 0; f < fm.y; f++) {
+                            const FluxDesc<real> &fd = s_flux[fm.x + f];
+                            if (r_raw > fd.cutoff) continue;
+                            real term = real(1) - r_raw / fd.cutoff;
+                            term = term * term;
+                            // Role of this particle in the entry (tf_flux_eval.h:64-67):
+                            // the type_a side; between equal types the lower id.
+                            const bool i_is_a = (ti == fd.type_a) && (ti != tj || id_i < id_j);
+                            const real sa = i_is_a ? spi[fd.index_a] : s_sp[k * S + fd.index_a];
+                            const real sb = i_is_a ? s_sp[k * S + fd.index_b] : spi[fd.index_b];
+                            real qv;
+                            if (fd.kind == FLUX_FICK) {
+                                qv = fd.coef * (sa - sb);
+                            } else if (fd.kind == FLUX_SECRETE) {
+                                qv = fd.coef * (sa - fd.target);
+                                qv = qv > real(0) ? qv : real(0);
+                            } else {
+                                qv = fd.coef * (fd.target - sb) * sa;
+                                qv = qv > real(0) ? qv : real(0);
+                            }
+                            qv *= term;
+                            if (i_is_a)
+                                qacc[fd.index_a] += -qv - fd.half_decay * sa;
+                            else
+                                qacc[fd.index_b] += qv - fd.half_decay * sb;
+                        }
+                    }
+                    if constexpr (!EXT) {
+                        // inside the table's range b (the FP64 decision for guarded pairs)
+                        const bool in = kMode == kFree ? ok && r_raw <= qd[0].y : (inmask & 1u) != 0;
+                        eval_term(qd, dx, dy, dz, r2, ir, r_raw, r_raw, in, id_j, k);
+                    } else {
+                        // The constituents in order; the CPU rule stops at the
+                        // first one in range (tf_potential_eval.h:211-215).
+                        const v4 *qt = qd;
+                        for (int t = 0; t < kMaxTerms; t++) {
+                            const v4 a0 = qt[0], a1 = qt[kDescPairs], a2 = qt[2 * kDescPairs];
+                            const int mt = real_bits_as_int(a1.w);
+                            real tdx = dx, tdy = dy, tdz = dz, tr2 = r2, tir = ir, traw = r_raw;
+                            image(qt, mt, tdx, tdy, tdz, tr2, tir, traw, k);
+                            const real rt = transformed(traw, mt, a2.x, k);
+                            const bool in = kMode == kFree ? ok && rt <= a0.y : ((inmask >> t) & 1u) != 0;
+                            eval_term(qt, tdx, tdy, tdz, tr2, tir, traw, rt, in, id_j, k);
+                            const int next = real_bits_as_int(a2.w);
+                            if (next == 0 || (in && !sd.sum_all)) break;
+                            qt = reinterpret_cast<const v4 *>(s_dyn + next * kCol);
+                        }
                     }
                 };
 
@@ -607,8 +686,8 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                                 k2[u] = min(base + ctz64_or_31(m0), klim);
                                 m0 &= m0 - 1ull;
                             }
-                            eval_pair(k2[0], ok2[0], kFreeMode);
-                            eval_pair(k2[1], ok2[1], kFreeMode);
+                            eval_pair(k2[0], ok2[0], kFreeMode, 0u);
+                            eval_pair(k2[1], ok2[1], kFreeMode, 0u);
                         }
                         // The deferred pairs, decided in FP64 (about one pair in
                         // 10^6), in candidate order: the round's block again, with
@@ -620,42 +699,48 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                                 const v4 pj = s_pos[k];
                                 const real r2 = dist2(pi4, pj);
                                 if (k == self_k || !(r2 <= cut2)) continue;
+                                const real dx = pi4.x - pj.x, dy = pi4.y - pj.y, dz = pi4.z - pj.z;
+                                const real ir = rsq(fmax(r2, eps2));
                                 const int tpb = ti_row + real_bits_as_int(pj.w);
                                 const v4 *qd = reinterpret_cast<const v4 *>(s_dyn + tpb);
-                                const v4 q0 = qd[0], q2 = qd[2 * kDescPairs];
-                                const int meta = real_bits_as_int(qd[kDescPairs].w);
-                                bool band = r2 > cut2_lo;
-                                if (binside) {
-                                    const real r_raw = r2 * rsq(fmax(r2, eps2));
-                                    real rt = r_raw;
-                                    if (RADIUS) {
-                                        if ((meta & kMetaScaled) && !(meta & kMetaDpd))
-                                            rt = r_raw / (pi4.w + s_rad[k]);
-                                        else if (meta & kMetaShifted)
-                                            rt = r_raw - (pi4.w + s_rad[k]) + ((meta & kMetaDpd) ? real(0) : q2.x);
-                                    }
-                                    band = band || ((meta & kMetaBInside) && fabs(rt - q0.y) <= real(2.0 * kGuard) * q0.y);
-                                }
-                                if (!band) continue;
-                                const double r2d = guard_r2(P.A.pos + s_slot[k], pi4, pj, sd.h, ihf);
+                                if (!in_band(qd, dx, dy, dz, r2, ir, r2 * ir, k)) continue;
+                                double d[3];
+                                guard_sep(P.A.pos + s_slot[k], pi4, pj, sd.h, ihf, d);
+                                const double r2d = __builtin_fma(d[2], d[2], __builtin_fma(d[1], d[1], d[0] * d[0]));
                                 if (!(r2d <= sd.cutoff2)) continue;
-                                const double bd = static_cast<double>(q2.y) + static_cast<double>(q2.z);
-                                bool inb;
-                                if (RADIUS && (meta & (kMetaScaled | kMetaShifted))) {
-                                    double rd = sqrt(r2d);
-                                    const double rs = static_cast<double>(pi4.w) + static_cast<double>(s_rad[k]);
-                                    if ((meta & kMetaScaled) && !(meta & kMetaDpd))
-                                        rd = rd / rs;
-                                    else
-                                        rd = rd - rs + ((meta & kMetaDpd) ? 0.0 : static_cast<double>(q2.x));
-                                    inb = rd <= bd;
-                                } else {
-                                    inb = r2d <= bd * bd;
+                                // each constituent's range in FP64 (its image
+                                // offset from the FP64 record, its radius
+                                // transform, its end b to 48 bits)
+                                unsigned inmask = 0;
+                                const v4 *qt = qd;
+                                int slot = tpb / kCol;
+                                for (int u = 0; u < kMaxTerms; u++) {
+                                    const v4 a1 = qt[kDescPairs], a2 = qt[2 * kDescPairs];
+                                    const int mt = real_bits_as_int(a1.w);
+                                    double e0 = d[0], e1 = d[1], e2 = d[2];
+                                    if (EXT && (mt & kMetaPeriodic)) {
+                                        const PotDesc<real> &pd = P.pots[P.slots[sd.pot_off + slot].x];
+                                        const double sg = mi.y < s_id[k] ? 1.0 : -1.0;
+                                        e0 -= sg * pd.offd[0];
+                                        e1 -= sg * pd.offd[1];
+                                        e2 -= sg * pd.offd[2];
+                                    }
+                                    double rd = sqrt(__builtin_fma(e2, e2, __builtin_fma(e1, e1, e0 * e0)));
+                                    if (EXT) {
+                                        const double rs = static_cast<double>(pi4.w) + static_cast<double>(s_rad[k]);
+                                        if ((mt & kMetaScaled) && !(mt & kMetaDpd))
+                                            rd = rd / rs;
+                                        else if (mt & kMetaShifted)
+                                            rd = rd - rs + ((mt & kMetaDpd) ? 0.0 : static_cast<double>(a2.x));
+                                    }
+                                    const double bd = static_cast<double>(a2.y) + static_cast<double>(a2.z);
+                                    if (rd <= bd) inmask |= 1u << u;
+                                    const int next = EXT ? real_bits_as_int(a2.w) : 0;
+                                    if (next == 0) break;
+                                    qt = reinterpret_cast<const v4 *>(s_dyn + next * kCol);
+                                    slot = next;
                                 }
-                                if (inb)
-                                    eval_pair(k, true, kDecidedInMode);
-                                else
-                                    eval_pair(k, true, kDecidedOutMode);
+                                eval_pair(k, true, kDecidedMode, inmask);
                             }
                         }
                     }
@@ -790,13 +875,13 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                                     }
                                     v4 p = P.A.pos[sj];
                                     const int2 mj = *reinterpret_cast<const int2 *>(&P.A.meta[sj]);
-                                    if (RADIUS) s_rad[nstaged + t] = p.w;
+                                    if (EXT) s_rad[nstaged + t] = p.w;
                                     p.x = real(static_cast<double>(p.x) + shx);
                                     p.y = real(static_cast<double>(p.y) + shy);
                                     p.z = real(static_cast<double>(p.z) + oz * sd.h[2]);
                                     p.w = as_real_bits(type_column<v4>(mj.x & 0xffff), real(0));
                                     s_pos[nstaged + t] = p;
-                                    if (kIds) s_id[nstaged + t] = mj.y;
+                                    if (kStageIds) s_id[nstaged + t] = mj.y;
                                     if (kGuarded) s_slot[nstaged + t] = sj;
                                     if (DPD) s_vel[nstaged + t] = P.A.vel[sj];
                                     if (FLUX)
@@ -887,14 +972,14 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
 // SIMD) once the compiler is asked for it: the only spill sits outside the
 // cell loops.  DPD, flux and FP64 variants need more registers and run at the
 // occupancy their register count gives.
-template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY>
+template <typename real, bool DPD, bool FLUX, bool EXT, bool CLDS, bool ENERGY>
 __global__ __launch_bounds__(kNbThreads) void k_nonbond(KP<real> P) {
-    nonbond_body<real, DPD, FLUX, RADIUS, CLDS, ENERGY>(P);
+    nonbond_body<real, DPD, FLUX, EXT, CLDS, ENERGY>(P);
 }
 
-template <bool RADIUS, bool CLDS, bool ENERGY>
+template <bool EXT, bool CLDS, bool ENERGY>
 __global__ __launch_bounds__(kNbThreads) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_nonbond_f32(KP<float> P) {
-    nonbond_body<float, false, false, RADIUS, CLDS, ENERGY>(P);
+    nonbond_body<float, false, false, EXT, CLDS, ENERGY>(P);
 }
 
 template <typename real>
@@ -902,21 +987,21 @@ __global__ __launch_bounds__(kNbThreads) void k_list_build(KP<real> P) {
     nonbond_body<real, false, false, false, false, false, true>(P);
 }
 
-template <typename real, bool DPD, bool FLUX, bool RADIUS, bool CLDS, bool ENERGY>
+template <typename real, bool DPD, bool FLUX, bool EXT, bool CLDS, bool ENERGY>
 static const void *nonbond_kernel() {
     if constexpr (std::is_same<real, float>::value && !DPD && !FLUX)
-        return reinterpret_cast<const void *>(&k_nonbond_f32<RADIUS, CLDS, ENERGY>);
+        return reinterpret_cast<const void *>(&k_nonbond_f32<EXT, CLDS, ENERGY>);
     else
-        return reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, RADIUS, CLDS, ENERGY>);
+        return reinterpret_cast<const void *>(&k_nonbond<real, DPD, FLUX, EXT, CLDS, ENERGY>);
 }
 
-template <typename real, bool DPD, bool FLUX, bool RADIUS>
+template <typename real, bool DPD, bool FLUX, bool EXT>
 static const void *nonbond_fn(bool clds, bool energy) {
     if (clds)
-        return energy ? nonbond_kernel<real, DPD, FLUX, RADIUS, true, true>()
-                      : nonbond_kernel<real, DPD, FLUX, RADIUS, true, false>();
-    return energy ? nonbond_kernel<real, DPD, FLUX, RADIUS, false, true>()
-                  : nonbond_kernel<real, DPD, FLUX, RADIUS, false, false>();
+        return energy ? nonbond_kernel<real, DPD, FLUX, EXT, true, true>()
+                      : nonbond_kernel<real, DPD, FLUX, EXT, true, false>();
+    return energy ? nonbond_kernel<real, DPD, FLUX, EXT, false, true>()
+                  : nonbond_kernel<real, DPD, FLUX, EXT, false, false>();
 }
 
 template <typename real> static const void *pick_nonbond(int key, bool clds, bool energy) {
@@ -933,7 +1018,7 @@ template <typename real> static const void *pick_nonbond(int key, bool clds, boo
 }
 
 static int variant_key(const DeviceEnsemble *E) {
-    return (E->has_dpd ? 4 : 0) | (E->has_flux ? 2 : 0) | (E->has_radius ? 1 : 0);
+    return (E->has_dpd ? 4 : 0) | (E->has_flux ? 2 : 0) | (E->has_ext ? 1 : 0);
 }
 
 template <typename real> int nonbond_setup(DeviceEnsemble *E, int max_nt, int max_coef) {
@@ -941,7 +1026,7 @@ template <typename real> int nonbond_setup(DeviceEnsemble *E, int max_nt, int ma
     E->coef_lds_bytes = (cbytes > 0 && cbytes <= static_cast<size_t>(kCoefLdsMax)) ? static_cast<int>(cbytes) : 0;
     const int max_chunk = 512;
     E->nb = nb_layout<real>(max_nt, E->coef_lds_bytes > 0 ? max_coef : 0, E->S, E->has_dpd, E->has_flux,
-                            E->has_radius, max_chunk);
+                            E->has_ext, max_chunk);
     // resident workgroups of the variant that runs every step (no energies)
     const void *fn = pick_nonbond<real>(variant_key(E), E->coef_lds_bytes > 0, false);
     int per_cu = 0;
@@ -984,9 +1069,6 @@ template <typename real> int run_list_build(DeviceEnsemble *E, const KP<real> &P
     return launch_error("list build");
 }
 
-
-__device__ inline float fma_of(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ inline double fma_of(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // FP64 squared distance of particles i and j from their stored positions and
 // current cells (the list kernel's cutoff guard): the cell offset is taken
@@ -1050,7 +1132,7 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     v4 *s_pair = reinterpret_cast<v4 *>(s_dyn);
     // build-cell shifts by offset code: (x, y) at 16 code, (z, 0) kListShiftZ bytes further
     double2 *s_sh = reinterpret_cast<double2 *>(s_dyn + P.nb.off_waves);
-    for (int t = threadIdx.x; t < nt * nt; t += kNbThreads) write_pair_descriptor<real, true>(P, sd, t, P.nb.off_coef, s_pair);
+    for (int t = threadIdx.x; t < sd.nslots; t += kNbThreads) write_pair_descriptor<real, true>(P, sd, t, P.nb.off_coef, s_pair);
     cache_tables(P, sd, s_dyn + P.nb.off_coef, P.nb.coef_half);
     for (int c = threadIdx.x; c < 125; c += kNbThreads) {
         s_sh[c] = make_double2((c / 25 - 2) * sd.h[0], (c / 5 % 5 - 2) * sd.h[1]);

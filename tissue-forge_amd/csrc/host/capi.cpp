@@ -139,6 +139,20 @@ int tfg_engine_set_integrator(tfg_engine *h, int kind) {
     }));
 }
 
+int tfg_engine_set_sum_rule(tfg_engine *h, int rule) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    if (rule != SUM_FIRST_IN_RANGE && rule != SUM_ALL) return set_error(TFG_E_INVALIDARG, "unknown sum rule");
+    TFG_GUARD(return with_refresh(h, [&] {
+        h->e.sum_rule = rule;
+        return TFG_OK;
+    }));
+}
+
+int tfg_engine_sum_rule(const tfg_engine *h) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
+    return h->e.sum_rule;
+}
+
 int tfg_engine_integrator(const tfg_engine *h) {
     if (!h) return set_error(TFG_E_INVALIDARG, "null engine");
     return h->e.integrator;
@@ -396,8 +410,9 @@ int tfg_potential_glj(tfg_potential **out, double e, double m, double n, double 
 int tfg_potential_from_table(tfg_potential **out, uint32_t flags, double a, double b, double r0_plusone,
                              const double alpha[3], int n, const double *c, int64_t ncoef) {
     if (!out || !alpha || (ncoef > 0 && !c)) return set_error(TFG_E_INVALIDARG, "null argument");
-    constexpr uint32_t kKnown = 0xFFFFu;   // PotentialFlags of tfPotential.h:56-108 (bits 0-15)
+    constexpr uint32_t kKnown = 0x1FFFFu;   // PotentialFlags of tfPotential.h:56-108 (bits 0-16)
     if (flags & ~kKnown) return set_error(TFG_E_INVALIDARG, "unknown potential flag bits");
+    if (flags & POT_SUM) return set_error(TFG_E_INVALIDARG, "a sum is made with tfg_potential_sum");
     if (n < 1 || n > kIntervalsMax) return set_error(TFG_E_INVALIDARG, "table interval count must be in [1, 3048]");
     if (ncoef != static_cast<int64_t>(kTableChunk) * (n + 1))
         return set_error(TFG_E_INVALIDARG, "a table of n intervals has 8 (n + 1) coefficients");
@@ -419,6 +434,54 @@ int tfg_potential_from_table(tfg_potential **out, uint32_t flags, double a, doub
             delete p;
             return set_error(TFG_E_INVALIDARG, why);
         }
+        *out = p;
+    });
+    return TFG_OK;
+}
+
+int tfg_potential_sum(tfg_potential **out, const tfg_potential *a, const tfg_potential *b) {
+    if (!out || !a || !b) return set_error(TFG_E_INVALIDARG, "null argument");
+    TFG_GUARD({
+        tfg_potential *p = new tfg_potential();
+        const int rc = create_sum(p->p, a->p, b->p);
+        if (rc != 0) {
+            delete p;
+            return set_error(TFG_E_INVALIDARG, rc == -1 ? "incompatible potentials (angle and non-angle)"
+                                                        : "a sum has at most 7 constituents");
+        }
+        *out = p;
+    });
+    return TFG_OK;
+}
+
+int tfg_potential_set_offset(tfg_potential *h, const double offset[3]) {
+    if (!h || !offset) return set_error(TFG_E_INVALIDARG, "null argument");
+    if (h->p.kind == KIND_SUM) return set_error(TFG_E_INVALIDARG, "set the offset of a constituent, not of a sum");
+    for (int k = 0; k < 3; k++)
+        if (!std::isfinite(offset[k])) return set_error(TFG_E_INVALIDARG, "offset must be finite");
+    for (int k = 0; k < 3; k++) h->p.offset[k] = offset[k];
+    h->p.flags |= POT_PERIODIC;
+    return TFG_OK;
+}
+
+int tfg_potential_offset(const tfg_potential *h, double offset[3]) {
+    if (!h || !offset) return set_error(TFG_E_INVALIDARG, "null argument");
+    for (int k = 0; k < 3; k++) offset[k] = h->p.offset[k];
+    return TFG_OK;
+}
+
+int tfg_potential_terms(const tfg_potential *h) {
+    if (!h) return set_error(TFG_E_INVALIDARG, "null potential");
+    return h->p.kind == KIND_SUM ? static_cast<int>(h->p.terms.size()) : 0;
+}
+
+int tfg_potential_term(const tfg_potential *h, int k, tfg_potential **out) {
+    if (!h || !out) return set_error(TFG_E_INVALIDARG, "null argument");
+    if (h->p.kind != KIND_SUM || k < 0 || k >= static_cast<int>(h->p.terms.size()))
+        return set_error(TFG_E_RANGE, "no such constituent");
+    TFG_GUARD({
+        tfg_potential *p = new tfg_potential();
+        p->p = h->p.terms[k];
         *out = p;
     });
     return TFG_OK;
@@ -454,6 +517,22 @@ int tfg_potential_coeffs(const tfg_potential *h, double *out, int cap) {
 
 int tfg_potential_eval(const tfg_potential *h, double r, double ri, double rj, double *e, double *f) {
     if (!h || !e || !f) return set_error(TFG_E_INVALIDARG, "null argument");
+    if (h->p.kind == KIND_SUM) {   // Potential::operator(): every tabulated constituent in range
+        double es = 0, fs = 0;
+        bool any = false;
+        for (const Potential &t : h->p.terms) {
+            double et, ft;
+            if (t.kind == KIND_TABLE && table_eval_ex(t, ri, rj, r * r, &et, &ft)) {
+                es += et;
+                fs += ft;
+                any = true;
+            }
+        }
+        if (!any) return set_error(TFG_E_RANGE, "distance outside every constituent's table");
+        *e = es;
+        *f = fs;
+        return TFG_OK;
+    }
     if (h->p.kind != KIND_TABLE) return set_error(TFG_E_INVALIDARG, "not a table potential");
     if (!table_eval_ex(h->p, ri, rj, r * r, e, f)) return set_error(TFG_E_RANGE, "distance outside the table");
     return TFG_OK;

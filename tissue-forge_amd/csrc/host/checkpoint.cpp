@@ -18,6 +18,7 @@
 // and "nan".  A save / load round trip is exact.
 #include <charconv>
 #include <cmath>
+#include <functional>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -346,6 +347,7 @@ std::string engine_to_json(const Engine &e) {
     w.raw(","); w.key("nr_fluxsteps"); w.integer(e.nr_fluxsteps);
     w.raw(","); w.key("max_distance_fraction"); w.num(e.max_dist_fraction);
     w.raw(","); w.key("integrator"); w.integer(e.integrator);
+    w.raw(","); w.key("sum_rule"); w.integer(e.sum_rule);
     w.raw(","); w.key("time"); w.integer(e.time);
     w.raw(","); w.key("nspecies"); w.integer(e.nspecies);
     w.raw(",\n"); w.key("dpd_noise");
@@ -378,10 +380,8 @@ std::string engine_to_json(const Engine &e) {
         w.raw("}");
     }
     w.raw("]");
-    w.raw(",\n"); w.key("potentials"); w.raw("[");
-    for (size_t k = 0; k < e.pots.size(); k++) {
-        const Potential &pt = *e.pots[k];
-        if (k) w.raw(",\n");
+    // A potential record; a sum carries its constituents as records of their own.
+    std::function<void(const Potential &)> write_pot = [&](const Potential &pt) {
         w.raw("{"); w.key("name"); w.str(pt.name);
         w.raw(","); w.key("kind"); w.integer(pt.kind);
         w.raw(","); w.key("flags"); w.integer(pt.flags);
@@ -392,8 +392,22 @@ std::string engine_to_json(const Engine &e) {
         w.raw(","); w.key("n"); w.integer(pt.n);
         const double dpd[3] = {pt.dpd_alpha, pt.dpd_gamma, pt.dpd_sigma};
         w.raw(","); w.key("dpd"); w.nums(dpd, 3);
+        w.raw(","); w.key("offset"); w.nums(pt.offset, 3);
         w.raw(","); w.key("c"); w.nums(pt.c.data(), pt.c.size());
+        if (pt.kind == KIND_SUM) {
+            w.raw(","); w.key("terms"); w.raw("[");
+            for (size_t t = 0; t < pt.terms.size(); t++) {
+                if (t) w.raw(",");
+                write_pot(pt.terms[t]);
+            }
+            w.raw("]");
+        }
         w.raw("}");
+    };
+    w.raw(",\n"); w.key("potentials"); w.raw("[");
+    for (size_t k = 0; k < e.pots.size(); k++) {
+        if (k) w.raw(",\n");
+        write_pot(*e.pots[k]);
     }
     w.raw("]");
     // Bindings (pot, ti, tj) with ti <= tj: bind() registers both orders.
@@ -522,6 +536,11 @@ int engine_from_json(Engine &e, const char *text, size_t len) {
         const long long integ = int_of(root, "integrator");
         if (integ != INTEG_FORWARD_EULER && integ != INTEG_RUNGE_KUTTA_4) throw ParseError{"unknown integrator"};
         e.integrator = static_cast<int>(integ);
+        if (root.get("sum_rule")) {
+            const long long rule = int_of(root, "sum_rule");
+            if (rule != SUM_FIRST_IN_RANGE && rule != SUM_ALL) throw ParseError{"unknown sum rule"};
+            e.sum_rule = static_cast<int>(rule);
+        }
         e.time = int_of(root, "time");
         const long long ns = int_of(root, "nspecies");
         if (ns < 0 || ns > 8) throw ParseError{"nspecies must be in [0, 8]"};
@@ -561,13 +580,14 @@ int engine_from_json(Engine &e, const char *text, size_t len) {
             if (rc < 0) return rc;
         }
         const int nt = static_cast<int>(e.types.size());
-        for (const JVal &pj : objs_of(root, "potentials")) {
+        std::function<Potential(const JVal &, bool)> read_pot = [&](const JVal &pj, bool term) -> Potential {
             Potential pt;
             const JVal &nm = need(pj, "name");
             if (nm.kind != JVal::STR) throw ParseError{"potential \"name\" must be a string"};
             pt.name = nm.s;
             const long long kind = int_of(pj, "kind");
-            if (kind != KIND_TABLE && kind != KIND_DPD) throw ParseError{"unknown potential kind"};
+            if (kind != KIND_TABLE && kind != KIND_DPD && (term || kind != KIND_SUM))
+                throw ParseError{"unknown potential kind"};
             pt.kind = static_cast<uint32_t>(kind);
             pt.flags = static_cast<uint32_t>(int_of(pj, "flags"));
             const auto &al = nums_of(pj, "alpha", 4);
@@ -584,9 +604,19 @@ int engine_from_json(Engine &e, const char *text, size_t len) {
             pt.dpd_sigma = dpd[2];
             const auto &c = nums_of(pj, "c", pt.kind == KIND_TABLE ? static_cast<long long>(kTableChunk) * (n + 1) : -1);
             pt.c = c;
+            if (pj.get("offset")) {
+                const auto &off = nums_of(pj, "offset", 3);
+                for (int k = 0; k < 3; k++) pt.offset[k] = off[k];
+            }
+            if (pt.kind == KIND_SUM)
+                for (const JVal &tj : objs_of(pj, "terms")) pt.terms.push_back(read_pot(tj, true));
             // what the device evaluation relies on (interval range, finite map and coefficients)
             const std::string why = check_potential(pt);
             if (!why.empty()) throw ParseError{"potential \"" + pt.name + "\": " + why};
+            return pt;
+        };
+        for (const JVal &pj : objs_of(root, "potentials")) {
+            Potential pt = read_pot(pj, false);
             // kept as saved (no de-duplication), so the saved ids stay valid
             e.pots.push_back(std::make_shared<Potential>(std::move(pt)));
         }

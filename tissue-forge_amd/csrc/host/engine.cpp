@@ -100,6 +100,10 @@ int Engine::set_wall_potential(int face, int type, int pot) {
     if (face < -1 || face > 5) return set_error(TFG_E_INVALIDARG, "face must be in [-1, 5]");
     if (type < 0 || type >= static_cast<int>(types.size())) return set_error(TFG_E_INVALIDARG, "unknown particle type");
     if (pot < -1 || pot >= static_cast<int>(pots.size())) return set_error(TFG_E_INVALIDARG, "unknown potential");
+    // boundary_eval calls potential_eval_ex on the wall potential itself
+    // (tf_boundary_eval.h:204-254): a single table or DPD record
+    if (pot >= 0 && (pots[pot]->kind == KIND_SUM || (pots[pot]->flags & POT_PERIODIC)))
+        return set_error(TFG_E_INVALIDARG, "a wall potential is a single table or DPD potential (no sum, no periodic image)");
     for (int f = 0; f < 6; f++)
         if (face < 0 || face == f) wallpot[static_cast<size_t>(f) * max_types + type] = pot;
     return TFG_OK;
@@ -142,6 +146,11 @@ static bool same_potential(const Potential &a, const Potential &b) {
         return false;
     for (int k = 0; k < 4; k++)
         if (a.alpha[k] != b.alpha[k]) return false;
+    for (int k = 0; k < 3; k++)
+        if (a.offset[k] != b.offset[k]) return false;
+    if (a.terms.size() != b.terms.size()) return false;
+    for (size_t k = 0; k < a.terms.size(); k++)
+        if (!same_potential(a.terms[k], b.terms[k])) return false;
     return true;
 }
 
@@ -238,7 +247,8 @@ int Engine::add_particles(int n, const int *tp, const double *x, const double *v
 
 int Engine::add_bonds(int pot, int n, const int32_t *ij) {
     if (device) return set_error(TFG_E_STATE, "engine is on the device; call from_device first");
-    if (pot < 0 || pot >= static_cast<int>(pots.size()) || pots[pot]->kind != KIND_TABLE)
+    if (pot < 0 || pot >= static_cast<int>(pots.size()) || pots[pot]->kind != KIND_TABLE ||
+        (pots[pot]->flags & POT_PERIODIC))
         return set_error(TFG_E_RANGE, "bond potential must be a table potential");
     const int np = static_cast<int>(parts.size());
     for (int b = 0; b < n; b++) {
@@ -252,7 +262,8 @@ int Engine::add_bonds(int pot, int n, const int32_t *ij) {
 
 int Engine::add_angles(int pot, int n, const int32_t *ijk) {
     if (device) return set_error(TFG_E_STATE, "engine is on the device; call from_device first");
-    if (pot < 0 || pot >= static_cast<int>(pots.size()) || pots[pot]->kind != KIND_TABLE)
+    if (pot < 0 || pot >= static_cast<int>(pots.size()) || pots[pot]->kind != KIND_TABLE ||
+        (pots[pot]->flags & POT_PERIODIC))
         return set_error(TFG_E_RANGE, "angle potential must be a table potential");
     const int np = static_cast<int>(parts.size());
     for (int a = 0; a < n; a++)

@@ -34,6 +34,11 @@ enum DpdNoise : int { DPD_NOISE_UNIT = 0, DPD_NOISE_PHILOX = 1 };
 // EngineIntegrator (tfEngine.h:89-92), chosen by the Universe configuration
 // (tfSimulator.cpp:218-229).
 enum Integrator : int { INTEG_FORWARD_EULER = 0, INTEG_RUNGE_KUTTA_4 = 1 };
+// How a nonbonded sum potential is evaluated: the constituents in order up to
+// the first one in range (the CPU engine: potential_eval_super_ex evaluates
+// `pca || pcb`, tf_potential_eval.h:211-215), or every constituent (the
+// reference's GPU path, tfRunner_cuda.cu:436-560, and Potential::operator()).
+enum SumRule : int { SUM_FIRST_IN_RANGE = 0, SUM_ALL = 1 };
 
 // BoundaryConditionKind (tfBoundaryConditions.h:38-46).
 enum BoundaryKind : unsigned {
@@ -135,6 +140,7 @@ class Engine {
     double dt_flux = 0.01;
     double max_dist_fraction = 0.05;   // tfEngine.cpp:1865
     int integrator = INTEG_FORWARD_EULER;
+    int sum_rule = SUM_FIRST_IN_RANGE;
     BoundaryFace bc_set[6];        // faces as set (every face periodic by default)
     // Neighbour lists with a skin (0: off); capacity in entries per particle
     // (0: from the density).  The pair set of every step is unchanged.

@@ -509,8 +509,37 @@ int create_glj(Potential &p, double e, double m, double n, double k, double r0, 
     return rc;
 }
 
+int create_sum(Potential &p, const Potential &a, const Potential &b) {
+    if ((a.flags & POT_ANGLE) && !(b.flags & POT_ANGLE)) return -1;   // "Incompatible potentials"
+    Potential s;
+    s.kind = KIND_SUM;
+    s.flags = POT_SUM;
+    for (const Potential *q : {&a, &b}) {
+        if (q->kind == KIND_SUM)
+            s.terms.insert(s.terms.end(), q->terms.begin(), q->terms.end());
+        else
+            s.terms.push_back(*q);
+    }
+    if (s.terms.size() > static_cast<size_t>(kMaxTerms)) return -3;
+    s.name = a.name + " PLUS " + b.name;
+    p = std::move(s);
+    return 0;
+}
+
 std::string check_potential(const Potential &p) {
     auto fin = [](double v) { return std::isfinite(v); };
+    for (int k = 0; k < 3; k++)
+        if (!fin(p.offset[k])) return "periodic offset must be finite";
+    if (p.kind == KIND_SUM) {
+        if (p.terms.size() < 2 || p.terms.size() > static_cast<size_t>(kMaxTerms))
+            return "a sum has 2 to 7 constituents";
+        for (const Potential &t : p.terms) {
+            if (t.kind == KIND_SUM) return "constituents of a sum are tables or DPD";
+            const std::string why = check_potential(t);
+            if (!why.empty()) return why;
+        }
+        return "";
+    }
     if (p.kind == KIND_DPD) {
         if (!fin(p.a) || !fin(p.b) || !(p.b > 0)) return "DPD range must be finite with a positive cutoff";
         if (!fin(p.dpd_alpha) || !fin(p.dpd_gamma) || !fin(p.dpd_sigma)) return "DPD coefficients must be finite";

@@ -26,6 +26,7 @@ constexpr int kIntervalsMax = 3048;  // tfPotential.h:45
 enum PotentialKind : uint32_t {
     KIND_TABLE = 0,   // POTENTIAL_KIND_POTENTIAL
     KIND_DPD = 1,     // POTENTIAL_KIND_DPD
+    KIND_SUM = 3,     // POTENTIAL_KIND_COMBINATION with POTENTIAL_SUM
 };
 
 // Subset of the reference's PotentialFlags (tfPotential.h:56-108) that
@@ -36,7 +37,14 @@ enum PotentialFlags : uint32_t {
     POT_HARMONIC = 1u << 7,
     POT_SCALED = 1u << 11,
     POT_SHIFTED = 1u << 12,
+    POT_SUM = 1u << 14,
+    POT_PERIODIC = 1u << 15,   // evaluated at dx - offset (a periodic image)
 };
+
+// Constituents of a sum the device evaluates per type pair: a base
+// potential and its images on both sides of three periodic axes, the sum
+// create_periodic_potential makes at order 1 (tfPotential.cpp:158-179).
+constexpr int kMaxTerms = 7;
 
 struct Potential {
     uint32_t kind = KIND_TABLE;
@@ -48,6 +56,12 @@ struct Potential {
     std::vector<double> c;   // kTableChunk * (n + 1)
     // DPD parameters (KIND_DPD only); a = sqrt(eps), b = cutoff.
     double dpd_alpha = 0, dpd_gamma = 0, dpd_sigma = 0;
+    // POT_PERIODIC: the image offset subtracted from the separation
+    // (potential_eval_super_ex, tf_potential_eval.h:220-225).
+    double offset[3] = {0, 0, 0};
+    // KIND_SUM: the constituents in order (Potential::constituents,
+    // tfPotential.cpp:2683-2700: nested sums flattened), each a table or DPD.
+    std::vector<Potential> terms;
     std::string name;
 };
 
@@ -85,6 +99,12 @@ int create_overlapping_sphere(Potential &p, double mu, double k, double harmonic
 // shifted (r - (ri + rj) + r0 + 1) or scaled (r / (ri + rj)).
 int create_glj(Potential &p, double e, double m, double n, double k, double r0,
                double min, double max, double tol, bool shifted);
+
+// Potential::operator+ (tfPotential.cpp:2706-2727): a sum of a and b with
+// their constituents flattened in order.  Fails (negative) when a is an
+// angle potential and b is not, or the sum has more than kMaxTerms
+// constituents.
+int create_sum(Potential &p, const Potential &a, const Potential &b);
 
 // Checks a table or DPD record for what the device evaluation relies on:
 // for a table 1 <= n <= kIntervalsMax, 8 (n + 1) finite coefficients, a

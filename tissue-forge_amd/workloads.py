@@ -34,6 +34,9 @@ Workloads (particle counts are the full sizes; every builder takes a scale):
                   r - (ri + rj) + r0 + 1), the generalized Lennard-Jones with
                   its defaults (scaled: r / (ri + rj)), an overlapping-sphere
                   potential (scaled) and a shifted DPD potential.
+  sum_mix         the radius_mix lattice with sums of potentials (table +
+                  table, DPD + table, periodic images + table, four terms)
+                  and a lone periodic potential, under either sum rule.
 """
 import math
 
@@ -195,6 +198,36 @@ def radius_mix(n=4000, density=0.5, seed=8, dt=0.002, cutoff=2.5, dpd=True, tol=
                 species=None, bonds=None, angles=None, dpd_noise=None)
 
 
+def sum_mix(n=4000, density=0.5, seed=9, dt=0.002, cutoff=2.5, rule=0):
+    """Sums of potentials (Potential::operator+) and periodic image
+    constituents (Potential::offset) between types of radii 0.3, 0.45 and
+    0.6, on the radius_mix lattice: a table sum whose ranges overlap, DPD
+    plus a table, two opposite periodic images plus a table, a four-term sum
+    and a lone periodic potential.  rule is the engine's sum rule."""
+    sc = radius_mix(n=n, density=density, seed=seed, dt=dt, cutoff=cutoff, dpd=False)
+    morse_s = dict(kind="morse", d=1.0, a=6.0, r0=0.0, min=0.0001, max=1.5, tol=0.001, shifted=True)
+    glj = dict(kind="glj", e=0.5, m=3.0, n=6.0, k=0.0, r0=1.0, min=0.05, max=2.5, tol=0.01, shifted=False)
+    dpd = dict(kind="dpd", alpha=10.0, gamma=1.0, sigma=1.0, cutoff=1.0, shifted=False)
+    morse_l = dict(kind="morse", d=0.3, a=2.0, r0=1.2, min=0.0001, max=2.4, tol=0.001, shifted=False)
+    img = (0.35, -0.2, 0.1)
+    per_p = dict(kind="morse", d=0.5, a=3.0, r0=0.8, min=0.0001, max=1.8, tol=0.001, shifted=False, offset=img)
+    per_m = dict(per_p, offset=tuple(-v for v in img))
+    lone = dict(kind="morse", d=0.4, a=4.0, r0=0.9, min=0.0001, max=2.0, tol=0.001, shifted=False,
+                offset=(0.0, 0.3, -0.25))
+    sc["pots"] = [
+        dict(kind="sum", terms=[morse_s, glj]),
+        dict(kind="sum", terms=[dpd, morse_l]),
+        dict(kind="sum", terms=[per_p, per_m, glj]),
+        dict(kind="sum", terms=[per_p, dpd, per_m, morse_l]),
+        lone,
+        morse_l,
+    ]
+    sc["bind"] = [(0, 0, 0), (1, 0, 1), (2, 1, 1), (3, 1, 2), (4, 2, 2), (5, 0, 2)]
+    sc["sum_rule"] = int(rule)
+    sc["name"] = "sum_mix"
+    return sc
+
+
 def ensemble(nrep=512, seed=5, nmin=500, nmax=20000, dt=0.005):
     rng = np.random.default_rng(seed)
     ns = np.rint(np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=nrep))).astype(int)
@@ -212,6 +245,23 @@ def ensemble(nrep=512, seed=5, nmin=500, nmax=20000, dt=0.005):
 
 
 def make_potential(p):
+    """A product Potential from a spec dict; "sum" specs list their
+    constituents under "terms", any non-sum spec may carry a periodic
+    "offset"."""
+    kind = p["kind"]
+    if kind == "sum":
+        terms = [make_potential(t) for t in p["terms"]]
+        out = terms[0]
+        for t in terms[1:]:
+            out = out + t
+        return out
+    pot = _make_single(p)
+    if p.get("offset") is not None:
+        pot.offset = p["offset"]
+    return pot
+
+
+def _make_single(p):
     kind = p["kind"]
     if kind == "morse":
         return Potential.morse(d=p["d"], a=p["a"], r0=p["r0"], min=p["min"], max=p["max"], tol=p["tol"],
@@ -253,6 +303,8 @@ def build_engine(scene, potentials=None, max_distance_fraction=None):
         eng.set_wall_potential(face, t, pots[pi])
     if scene.get("dpd_noise"):
         eng.set_dpd_noise(*scene["dpd_noise"])
+    if scene.get("sum_rule"):
+        eng.sum_rule = scene["sum_rule"]
     if scene.get("verlet_skin"):
         eng.set_verlet(scene["verlet_skin"], scene.get("verlet_capacity", 0))
     eng.add_particles(scene["ptype"], scene["x"], scene["v"], scene["species"])
