@@ -325,6 +325,22 @@ int tfg_engine_step(tfg_engine *e, int nsteps);
  * and bonded forces, no update.  The role of cuda::engine_nonbond_cuda
  * (source/mdcore/src/tfRunner_cuda.cu:2940-3354) plus the bonded pass. */
 int tfg_nonbond_eval(tfg_engine *e);
+/* The pass of tfg_nonbond_eval for an engine on the host, in one call that
+ * allocates no device memory: upload, binning, nonbonded and bonded forces
+ * and the energies, with the forces and number density of each particle by
+ * id into f (n*4) and the six values of tfg_read_energies into energies (may
+ * be NULL).  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Every device buffer of the pass is carved from one fixed
+ * 64 MiB array that each GPU the library runs on holds until the process
+ * ends: about 300,000 particles in mixed precision, 200,000 in FP64 (fewer
+ * with species or bonds); a larger system fails with TFG_E_RANGE before any
+ * GPU work.  The pass is the cell kernel (neighbour-list settings do not
+ * apply) on the null stream; the engine's state is unchanged and the call
+ * returns after the GPU work has drained.  Calls on one GPU share the
+ * array, so a per-device lock makes overlapping calls from several threads
+ * run one after the other.  TFG_E_STATE while the engine is on the device. */
+int tfg_nonbond_eval_noalloc(tfg_engine *e, int precision, double *f, double energies[6]);
 
 /* ---- readback (synchronous). ---- */
 /* out[0..5] = epot, epot_nonbond, epot_bond, epot_angle, ekin, virial of the

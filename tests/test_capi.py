@@ -79,6 +79,9 @@ def test_to_device_fails_loudly_without_gpu(tf):
         eng.to_device(tf.PREC_MIXED)
     assert ei.value.code == E_DEVICE
     assert not eng.on_device
+    with pytest.raises(tf.TfgError) as ei:
+        eng.force_noalloc(tf.PREC_MIXED)
+    assert ei.value.code == E_DEVICE
 
 
 def test_geometry_and_insertion(tf):

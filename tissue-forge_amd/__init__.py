@@ -48,6 +48,7 @@ _lib = None
 
 
 E_STATE = -0x7FFF0001   # TFG_E_STATE (0x8000FFFF as a signed 32-bit code)
+E_RANGE = -0x7FFFFFF5   # TFG_E_RANGE (0x8000000B)
 
 
 class TfgError(RuntimeError):
@@ -134,6 +135,7 @@ _SIGNATURES = {
     "tfg_engine_launch_config": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "tfg_engine_step": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tfg_nonbond_eval": (ctypes.c_int, [_vp]),
+    "tfg_nonbond_eval_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "tfg_read_energies": (ctypes.c_int, [_vp, _dp]),
     "tfg_read_particles": (ctypes.c_int, [_vp, _i32p, _dp, _dp, _dp, _dp, _dp]),
     "tfg_read_positions": (ctypes.c_int, [_vp, _dp]),
@@ -603,6 +605,14 @@ class Engine:
 
     def force(self):
         _check(lib().tfg_nonbond_eval(self._h))
+
+    def force_noalloc(self, precision=PREC_MIXED):
+        """The force pass for an engine on the host in one call that
+        allocates no device memory; returns (f, energies) with f (n, 4) by id."""
+        f = np.zeros((self.nparts, 4))
+        e = np.zeros(6)
+        _check(lib().tfg_nonbond_eval_noalloc(self._h, int(precision), _dptr(f), _dptr(e)))
+        return f, dict(zip(["epot", "epot_nonbond", "epot_bond", "epot_angle", "ekin", "virial"], e.tolist()))
 
     def last_step_ms(self):
         return lib().tfg_last_step_ms(self._h)

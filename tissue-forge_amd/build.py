@@ -16,6 +16,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "device", "bin.hip"),
     os.path.join(HERE, "csrc", "device", "nonbond.hip"),
     os.path.join(HERE, "csrc", "device", "advance.hip"),
+    os.path.join(HERE, "csrc", "device", "noalloc.hip"),
     os.path.join(HERE, "csrc", "host", "capi.cpp"),
     os.path.join(HERE, "csrc", "host", "engine.cpp"),
     os.path.join(HERE, "csrc", "host", "potential.cpp"),

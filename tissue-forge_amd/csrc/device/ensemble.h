@@ -20,14 +20,27 @@ namespace tfg {
             return set_error(TFG_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Device buffers of one ensemble: hipMalloc'd one by one, or carved in
+// 256-byte steps from a caller's arena (no allocation; nothing to free).
 struct Buffers {
     std::vector<void *> ptrs;
+    unsigned char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
     ~Buffers() {
         for (void *p : ptrs) (void)hipFree(p);
     }
     template <typename T> int alloc(T **out, size_t n) {
-        void *p = nullptr;
         if (n == 0) n = 1;
+        if (arena) {
+            const size_t bytes = (n * sizeof(T) + 255) & ~static_cast<size_t>(255);
+            if (bytes > arena_cap - arena_used)
+                return set_error(TFG_E_RANGE, "the system needs more device memory than the " +
+                                                  std::to_string(arena_cap >> 20) + " MiB scratch array holds");
+            *out = reinterpret_cast<T *>(arena + arena_used);
+            arena_used += bytes;
+            return TFG_OK;
+        }
+        void *p = nullptr;
         TFG_HIP(hipMalloc(&p, n * sizeof(T)));
         ptrs.push_back(p);
         *out = static_cast<T *>(p);
@@ -61,6 +74,8 @@ struct DeviceEnsemble {
     NbLayout nbb{};           // LDS layout of the list-build variant
     NbLayout nbl{};           // LDS of the list pair kernel (off_coef: tables, off_waves: cell offsets)
     bool list_mode = false;   // neighbour lists in use (every member asked for them and can use them)
+    // no_alloc: buffers from an arena, the null stream, no events, no lists
+    bool no_alloc = false;
     int lcap = 0;
     int nlblocks = 0;
     int4 *d_lblock = nullptr;
@@ -127,6 +142,13 @@ struct DeviceEnsemble {
 int launch_error(const char *what);
 
 template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy);
+
+// engine_force for one host engine with every device buffer carved from
+// arena (cap bytes): upload, binning, nonbonded and bonded passes, energies;
+// forces and number density by particle id into f (n*4).  The engine's own
+// state is not changed.  Allocates no device memory, creates no stream or
+// event; runs on the null stream and returns after it has drained.
+int ensemble_force_arena(Engine *e, int precision, void *arena, size_t cap, double *f, Energies *en);
 
 // Binning: scan of the cell counts, placement, per-cell order, permutation
 // of buffer B into buffer A.  Cell counts come from the previous advance.

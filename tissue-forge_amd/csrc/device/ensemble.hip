@@ -247,9 +247,11 @@ int ensemble_create(DeviceEnsemble **out, Engine *const *engines, int n, int pre
 template <typename real> static int upload(DeviceEnsemble *E) {
     using v4 = typename V<real>::v4;
     TFG_HIP(hipGetDevice(&E->device));
-    TFG_HIP(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
-    for (int k = 0; k < 8; k++) TFG_HIP(hipEventCreate(&E->ev[k]));
-    E->nev = 8;
+    if (!E->no_alloc) {
+        TFG_HIP(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+        for (int k = 0; k < 8; k++) TFG_HIP(hipEventCreate(&E->ev[k]));
+        E->nev = 8;
+    }
     hipDeviceProp_t prop;
     TFG_HIP(hipGetDeviceProperties(&prop, E->device));
     E->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -500,7 +502,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     for (const SysDesc &d : sys) max_nt = std::max(max_nt, d.ntypes);
     // Lists carry tables only (no DPD, flux or radius transforms), with every
     // system's tables in LDS, and slots below 2^25.
-    E->list_mode = list_ok && !E->has_dpd && !E->has_flux && !E->has_ext && np + 1 < (1 << kListSlotBits) &&
+    E->list_mode = list_ok && !E->no_alloc && !E->has_dpd && !E->has_flux && !E->has_ext && np + 1 < (1 << kListSlotBits) &&
                    static_cast<size_t>(max_coef) * sizeof(real) <= 16384 && max_coef > 0;
     E->lcap = E->list_mode ? lcap : 0;
     int rc = nonbond_setup<real>(E, max_nt, max_coef);
@@ -913,27 +915,69 @@ template <typename real> static int force_t(DeviceEnsemble *E) {
     KP<real> P = make_params<real>(E, 1);
     P.step_ahead = 1;   // the evaluation belongs to the next step (engine_step increments time first)
     int rc;
+    const bool timed = E->nev > 0;
     if (E->list_mode)
         hipLaunchKernelGGL(k_list_rebin, dim3(1), dim3(256), 0, E->stream, E->d_lflags, E->d_sflags, E->nsys);
-    TFG_HIP(hipEventRecord(E->ev[0], E->stream));
+    if (timed) TFG_HIP(hipEventRecord(E->ev[0], E->stream));
     if ((rc = run_bin<real>(E, P)) != TFG_OK) return rc;
-    TFG_HIP(hipEventRecord(E->ev[1], E->stream));
+    if (timed) TFG_HIP(hipEventRecord(E->ev[1], E->stream));
     // list mode: the lists of the new order (timed for the autotuner, which
     // tunes the build's grid); the next step rebuilds again
     if (E->list_mode && (rc = run_list_build<real>(E, P)) != TFG_OK) return rc;
-    TFG_HIP(hipEventRecord(E->ev[2], E->stream));
+    if (timed) TFG_HIP(hipEventRecord(E->ev[2], E->stream));
     if ((rc = run_nonbond<real>(E, P)) != TFG_OK) return rc;
-    TFG_HIP(hipEventRecord(E->ev[3], E->stream));
+    if (timed) TFG_HIP(hipEventRecord(E->ev[3], E->stream));
     if ((rc = run_advance<real>(E, P, 0)) != TFG_OK) return rc;
-    TFG_HIP(hipEventRecord(E->ev[4], E->stream));
+    if (timed) TFG_HIP(hipEventRecord(E->ev[4], E->stream));
     if ((rc = run_energy<real>(E, P)) != TFG_OK) return rc;
     TFG_HIP(hipStreamSynchronize(E->stream));
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < 4 && timed; k++) {
         float ms = 0;
         TFG_HIP(hipEventElapsedTime(&ms, E->ev[k], E->ev[k + 1]));
         E->kernel_ms[k] = ms;
     }
     return TFG_OK;
+}
+
+template <typename real>
+static int force_arena_t(Engine *e, void *arena, size_t cap, double *f, Energies *en) {
+    using v4 = typename V<real>::v4;
+    DeviceEnsemble E;
+    E.no_alloc = true;
+    E.buf.arena = static_cast<unsigned char *>(arena);
+    E.buf.arena_cap = cap;
+    E.precision = std::is_same<real, float>::value ? PREC_MIXED : PREC_FP64;
+    E.engines.assign(1, e);
+    E.integrator = e->integrator;
+    E.nr_fluxsteps = e->nr_fluxsteps;
+    int rc = upload<real>(&E);
+    if (rc != TFG_OK) return rc;
+    // the launch settings made off the device (setBlocks / setThreads), as
+    // the upload of tfg_engine_to_device applies them
+    if ((e->launch_blocks > 0 || e->launch_threads > 0) &&
+        (rc = ensemble_set_launch(&E, e->launch_blocks, e->launch_threads)) != TFG_OK)
+        return rc;
+    if ((rc = force_t<real>(&E)) != TFG_OK) return rc;
+    const int np = E.np;
+    std::vector<v4> hf(np);
+    std::vector<int> hslot(np);
+    TFG_HIP(hipMemcpy(hf.data(), E.force, sizeof(v4) * np, hipMemcpyDeviceToHost));
+    TFG_HIP(hipMemcpy(hslot.data(), E.slot_of, sizeof(int) * np, hipMemcpyDeviceToHost));
+    for (int p = 0; p < np; p++) {
+        const v4 q = hf[hslot[p]];
+        f[4 * p + 0] = q.x;
+        f[4 * p + 1] = q.y;
+        f[4 * p + 2] = q.z;
+        f[4 * p + 3] = q.w;
+    }
+    return ensemble_energies(&E, 0, en);
+}
+
+int ensemble_force_arena(Engine *e, int precision, void *arena, size_t cap, double *f, Energies *en) {
+    if (precision != PREC_MIXED && precision != PREC_FP64) return set_error(TFG_E_INVALIDARG, "unknown precision");
+    if (e->device) return set_error(TFG_E_STATE, "engine is on the device; call from_device first");
+    return precision == PREC_MIXED ? force_arena_t<float>(e, arena, cap, f, en)
+                                   : force_arena_t<double>(e, arena, cap, f, en);
 }
 
 int ensemble_force(DeviceEnsemble *E) {

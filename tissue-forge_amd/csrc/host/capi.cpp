@@ -668,6 +668,25 @@ int tfg_nonbond_eval(tfg_engine *h) {
     TFG_GUARD(return ensemble_force(h->e.device));
 }
 
+int tfg_nonbond_eval_noalloc(tfg_engine *h, int precision, double *f, double energies[6]) {
+    if (!h || !f) return set_error(TFG_E_INVALIDARG, "null argument");
+    if (h->e.device) return set_error(TFG_E_STATE, "engine is on the device; call from_device first");
+    Energies en;
+    TFG_GUARD({
+        const int rc = nonbond_eval_no_alloc(&h->e, precision, f, &en);
+        if (rc != TFG_OK) return rc;
+    });
+    if (energies) {
+        energies[0] = en.epot;
+        energies[1] = en.epot_nonbond;
+        energies[2] = en.epot_bond;
+        energies[3] = en.epot_angle;
+        energies[4] = en.ekin;
+        energies[5] = en.virial;
+    }
+    return TFG_OK;
+}
+
 int tfg_read_energies(tfg_engine *h, double out[6]) {
     if (!h || !out) return set_error(TFG_E_INVALIDARG, "null argument");
     Energies en = h->e.energies;

@@ -29,6 +29,11 @@ int ensemble_step(DeviceEnsemble *ens, int nsteps, bool use_graph);
 // Binning + nonbonded + bonded force evaluation only (no update).
 int ensemble_force(DeviceEnsemble *ens);
 int ensemble_energies(DeviceEnsemble *ens, int sys, Energies *out);
+// engine_force for a host engine in one call that allocates no device
+// memory: every buffer comes from the current GPU's fixed scratch array
+// (scratch_bytes() bytes).  Forces and number density by id into f (n*4).
+int nonbond_eval_no_alloc(Engine *e, int precision, double *f, Energies *en);
+size_t scratch_bytes();
 // FP64 per-species totals of one system after the last step; returns count.
 int ensemble_species_totals(DeviceEnsemble *ens, int sys, double *out, int cap);
 int ensemble_set_launch(DeviceEnsemble *ens, int blocks, int threads);
