@@ -61,6 +61,7 @@
 #include <cstring>
 #include <limits>
 #include <mutex>
+#include <numeric>
 #include <thread>
 #include <vector>
 
@@ -683,8 +684,8 @@ int build_space(System &s, const double origin[3], const double dim[3], const in
 }
 
 // Nonbonded pass: sort tasks, then self/pair tasks.  With several threads,
-// a task runs only while both of its cells are locked (the taboo scheme of
-// tfRunner.cpp:204-380 / tfQueue.cpp:109-163).
+// a task runs only while both of its cells are held (the rule of the taboo
+// scheme of tfRunner.cpp:204-380 / tfQueue.cpp:109-163).
 void nonbond(System &s, int nthreads) {
     const double self_nd = w_cubic(0.0, s.cutoff);
     for (Cell &c : s.cells) {
@@ -716,64 +717,87 @@ void nonbond(System &s, int nthreads) {
             });
         for (auto &t : th) t.join();
     }
+    // Threads claim chunks of consecutive tasks and run each task while
+    // holding both of its cells (the taboo rule), taking the two cell locks
+    // in index order so that no two threads wait on each other.
     const int nt = (int)s.tasks.size();
-    std::vector<std::atomic<char>> taboo(s.ncells);
-    for (auto &t : taboo) t.store(0);
-    std::vector<std::atomic<char>> done(nt);
-    for (auto &d : done) d.store(0);
-    std::atomic<int> first{0}, remaining{nt};
-    std::vector<double> vir(nthreads, 0.0);
+    std::vector<std::atomic<char>> held(s.ncells);
+    for (auto &h : held) h.store(0);
+    auto lock = [&](int c) {
+        char z = 0;
+        while (!held[c].compare_exchange_weak(z, 1, std::memory_order_acquire)) {
+            z = 0;
+            std::this_thread::yield();
+        }
+    };
+    auto unlock = [&](int c) { held[c].store(0, std::memory_order_release); };
+    // Consecutive claims go to chunks far apart in the task list (a stride
+    // coprime with the chunk count), so that concurrent chunks rarely share
+    // cells.
+    constexpr int kChunk = 32;
+    const long long nchunks = (nt + kChunk - 1) / kChunk;
+    long long stride = 7919;
+    while (nchunks > 1 && std::gcd(stride, nchunks) != 1) stride += 2;
+    std::atomic<long long> next{0};
+    struct alignas(64) Acc {
+        double v = 0.0;
+    };
+    std::vector<Acc> vir(nthreads);   // one cache line per thread
     std::vector<std::thread> th;
     for (int t = 0; t < nthreads; t++)
         th.emplace_back([&, t]() {
-            while (remaining.load() > 0) {
-                bool got = false;
-                for (int k = first.load(); k < nt; k++) {
-                    if (done[k].load()) {
-                        if (k == first.load()) first.compare_exchange_strong(k, k + 1);
-                        continue;
-                    }
+            double &my_vir = vir[t].v;
+            for (long long q = next++; q < nchunks; q = next++) {
+                const int k0 = static_cast<int>((q * stride) % nchunks) * kChunk;
+                for (int k = k0; k < std::min(nt, k0 + kChunk); k++) {
                     const Task &tk = s.tasks[k];
-                    char z = 0;
-                    if (!taboo[tk.i].compare_exchange_strong(z, 1)) continue;
-                    if (tk.type == 1) {
-                        z = 0;
-                        if (!taboo[tk.j].compare_exchange_strong(z, 1)) {
-                            taboo[tk.i].store(0);
-                            continue;
-                        }
-                    }
-                    char nd = 0;
-                    if (!done[k].compare_exchange_strong(nd, 1)) {
-                        taboo[tk.i].store(0);
-                        if (tk.type == 1) taboo[tk.j].store(0);
+                    if (tk.type == 0) {
+                        lock(tk.i);
+                        do_self(s, s.cells[tk.i], my_vir);
+                        unlock(tk.i);
                         continue;
                     }
-                    if (tk.type == 0)
-                        do_self(s, s.cells[tk.i], vir[t]);
-                    else
-                        do_pair(s, tk.i, tk.j, vir[t]);
-                    taboo[tk.i].store(0);
-                    if (tk.type == 1) taboo[tk.j].store(0);
-                    remaining--;
-                    got = true;
-                    break;
+                    const int a = std::min(tk.i, tk.j), b = std::max(tk.i, tk.j);
+                    lock(a);
+                    if (b != a) lock(b);
+                    do_pair(s, tk.i, tk.j, my_vir);
+                    if (b != a) unlock(b);
+                    unlock(a);
                 }
-                if (!got) std::this_thread::yield();
             }
         });
     for (auto &t : th) t.join();
-    for (double v : vir) vir_total += v;
+    for (const Acc &v : vir) vir_total += v.v;
     s.virial = vir_total;
 }
 
 // bond_eval (tfBond.cpp:97-345), no decay or dissociation.
-void bonds_eval(System &s, double &epot_out) {
-    double epot = 0.0;
+// Runs f(k) for k in [0, n) on nthreads threads (chunks of consecutive k).
+template <class F> void parallel_for(int n, int nthreads, F f) {
+    if (nthreads <= 1 || n < 1024) {
+        for (int k = 0; k < n; k++) f(k);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++)
+        th.emplace_back([&]() {
+            for (int k0 = next.fetch_add(256); k0 < n; k0 = next.fetch_add(256))
+                for (int k = k0; k < std::min(n, k0 + 256); k++) f(k);
+        });
+    for (auto &t : th) t.join();
+}
+
+// bond_eval (tfBond.cpp:97-345), no decay or dissociation.  Each bond's
+// force and energy are computed in parallel, then applied in bond order (the
+// sums are those of the serial loop).
+void bonds_eval(System &s, double &epot_out, int nthreads) {
     const int nb = (int)s.bonds.size() / 3;
-    for (int b = 0; b < nb; b++) {
-        Part &pi = s.part(s.bonds[3 * b]);
-        Part &pj = s.part(s.bonds[3 * b + 1]);
+    std::vector<double> w(4 * (size_t)nb);
+    std::vector<char> in(nb, 0);
+    parallel_for(nb, nthreads, [&](int b) {
+        const Part &pi = s.part(s.bonds[3 * b]);
+        const Part &pj = s.part(s.bonds[3 * b + 1]);
         const Pot &pot = s.pots[s.bonds[3 * b + 2]];
         const int *li = s.cells[s.where[pi.id].first].loc, *lj = s.cells[s.where[pj.id].first].loc;
         double pix[3], dx[3];
@@ -794,28 +818,38 @@ void bonds_eval(System &s, double &epot_out) {
         else if (pot.flags & POT_SHIFTED)
             r = r - (pi.radius + pj.radius) + pot.r0p1;
         double rr2 = r * r;
-        if (rr2 > pot.b * pot.b) continue;
+        if (rr2 > pot.b * pot.b) return;
         rr2 = std::fmax(rr2, pot.a * pot.a);
         double e, eff;
         eval_plain(pot, rr2, e, eff);
+        for (int k = 0; k < 3; k++) w[4 * (size_t)b + k] = eff * dx[k];
+        w[4 * (size_t)b + 3] = e;
+        in[b] = 1;
+    });
+    double epot = 0.0;
+    for (int b = 0; b < nb; b++) {
+        if (!in[b]) continue;
+        Part &pi = s.part(s.bonds[3 * b]);
+        Part &pj = s.part(s.bonds[3 * b + 1]);
         for (int k = 0; k < 3; k++) {
-            const double w = eff * dx[k];
-            pi.f[k] -= w;
-            pj.f[k] += w;
+            pi.f[k] -= w[4 * (size_t)b + k];
+            pj.f[k] += w[4 * (size_t)b + k];
         }
-        epot += e;
+        epot += w[4 * (size_t)b + 3];
     }
     epot_out += epot;
 }
 
-// angle_eval (tfAngle.cpp:90-384).
-void angles_eval(System &s, double &epot_out) {
-    double epot = 0.0;
+// angle_eval (tfAngle.cpp:90-384), computed in parallel and applied in
+// angle order like the bonds.
+void angles_eval(System &s, double &epot_out, int nthreads) {
     const int na = (int)s.angles.size() / 4;
-    for (int a = 0; a < na; a++) {
-        Part &pi = s.part(s.angles[4 * a]);
-        Part &pj = s.part(s.angles[4 * a + 1]);
-        Part &pk = s.part(s.angles[4 * a + 2]);
+    std::vector<double> w(7 * (size_t)na);
+    std::vector<char> in(na, 0);
+    parallel_for(na, nthreads, [&](int a) {
+        const Part &pi = s.part(s.angles[4 * a]);
+        const Part &pj = s.part(s.angles[4 * a + 1]);
+        const Part &pk = s.part(s.angles[4 * a + 2]);
         const Pot &pot = s.pots[s.angles[4 * a + 3]];
         const int *li = s.cells[s.where[pi.id].first].loc, *lj = s.cells[s.where[pj.id].first].loc,
                   *lk = s.cells[s.where[pk.id].first].loc;
@@ -848,16 +882,29 @@ void angles_eval(System &s, double &epot_out) {
             dxi[k] = (rjk[k] * injk - ct * rji[k] * inji) * inji;
             dxk[k] = (rji[k] * inji - ct * rjk[k] * injk) * injk;
         }
-        if (ct > pot.b) continue;
+        if (ct > pot.b) return;
         double e, eff;
-        if (!eval_r(pot, std::fmax(ct, pot.a), e, eff)) continue;
+        if (!eval_r(pot, std::fmax(ct, pot.a), e, eff)) return;
         for (int k = 0; k < 3; k++) {
-            const double wi = eff * dxi[k], wk = eff * dxk[k];
+            w[7 * (size_t)a + k] = eff * dxi[k];
+            w[7 * (size_t)a + 3 + k] = eff * dxk[k];
+        }
+        w[7 * (size_t)a + 6] = e;
+        in[a] = 1;
+    });
+    double epot = 0.0;
+    for (int a = 0; a < na; a++) {
+        if (!in[a]) continue;
+        Part &pi = s.part(s.angles[4 * a]);
+        Part &pj = s.part(s.angles[4 * a + 1]);
+        Part &pk = s.part(s.angles[4 * a + 2]);
+        for (int k = 0; k < 3; k++) {
+            const double wi = w[7 * (size_t)a + k], wk = w[7 * (size_t)a + 3 + k];
             pi.f[k] -= wi;
             pk.f[k] -= wk;
             pj.f[k] += wi + wk;
         }
-        epot += e;
+        epot += w[7 * (size_t)a + 6];
     }
     epot_out += epot;
 }
@@ -983,8 +1030,8 @@ void force(System &s, int nthreads) {
     s.reindex();
     s.epot_bond = 0;
     s.epot_angle = 0;
-    bonds_eval(s, s.epot_bond);
-    angles_eval(s, s.epot_angle);
+    bonds_eval(s, s.epot_bond, nthreads);
+    angles_eval(s, s.epot_angle, nthreads);
     s.epot_nb = 0;
     for (const Cell &c : s.cells) s.epot_nb += c.epot;
     s.epot = s.epot_nb + s.epot_bond + s.epot_angle;
