@@ -150,14 +150,22 @@ def test_list_mixed_matches_cell_mixed(tf, W, gpu):
 
 
 def test_list_fallbacks_and_ensemble(tf, W, O, gpu):
-    """Systems the lists do not cover step with the cell pass; an ensemble
-    of list systems matches each alone bitwise."""
-    dpd = W.dpd_droplets(n=6000)
-    dpd["verlet_skin"] = 0.2
-    eng, _ = W.build_engine(dpd)
+    """Systems the lists do not cover (radius transforms, Runge-Kutta 4)
+    step with the cell pass; an ensemble of list systems matches each alone
+    bitwise."""
+    rad = W.radius_mix(n=3000)
+    rad["verlet_skin"] = 0.3
+    eng, _ = W.build_engine(rad)
     eng.to_device(1)
     assert not eng.verlet_state()[0]
     eng.step(2)
+    eng.from_device()
+    rk = W.cell_sorting(n=2000, L=12.6, cells=4)
+    rk["verlet_skin"] = 0.3
+    rk["integrator"] = tf.RUNGE_KUTTA_4
+    eng, _ = W.build_engine(rk)
+    eng.to_device(1)
+    assert not eng.verlet_state()[0]
     eng.from_device()
     scenes = W.ensemble(nrep=4, nmin=3000, nmax=8000)
     for sc in scenes:
@@ -177,3 +185,59 @@ def test_list_fallbacks_and_ensemble(tf, W, O, gpu):
     for e, a in zip(engines, alone):
         p = e.particles()
         assert np.array_equal(p["cid"], a["cid"]) and np.array_equal(p["x"], a["x"])
+
+
+def _dpd_scene(W, n=6000, noise="philox", fluxsteps=3):
+    sc = W.dpd_droplets(n=n, noise=noise)
+    sc["nr_fluxsteps"] = fluxsteps
+    sc["verlet_skin"] = 0.3
+    return sc
+
+
+@pytest.mark.parametrize("fluxsteps", [1, 3])
+def test_list_dpd_flux_trajectory_fp64(tf, W, O, gpu, fluxsteps):
+    """DPD (Philox noise) and a Fick flux through the lists, with flux
+    sub-steps (the flux-only list pass): ten FP64 steps track the oracle."""
+    scene = _dpd_scene(W, fluxsteps=fluxsteps)
+    eng, pots = W.build_engine(scene)
+    orc = O.Oracle(scene, pots)
+    eng.to_device(1)
+    assert eng.verlet_state()[0]
+    eng.step(10)
+    assert orc.step(10, 4) == 0
+    pg, po = eng.particles(), orc.particles()
+    eg, eo = eng.energies(), orc.energies()
+    eng.from_device()
+    print("dpd lists fluxsteps %d: dx %.3e ds %.3e" % (fluxsteps, np.abs(pg["x"] - po["x"]).max(),
+                                                     np.abs(pg["species"] - po["species"]).max()))
+    assert np.array_equal(pg["cid"], po["cid"])
+    assert np.abs(pg["x"] - po["x"]).max() <= 1e-9
+    assert np.abs(pg["v"] - po["v"]).max() <= 1e-7 * max(1.0, np.abs(po["v"]).max())
+    assert np.abs(pg["species"] - po["species"]).max() <= 1e-10
+    assert rel(eg["ekin"], eo["ekin"]) <= 1e-9
+
+
+@pytest.mark.parametrize("precision,tol", [(1, 1e-10), (0, 1e-5)])
+def test_list_dpd_flux_first_step(tf, W, O, gpu, precision, tol):
+    """The first step's DPD forces, number density and species through the
+    lists against the oracle at the positions the path holds."""
+    scene = _dpd_scene(W, n=20000, fluxsteps=2)
+    eng, pots = W.build_engine(scene)
+    eng.to_device(precision)
+    assert eng.verlet_state()[0]
+    xq = eng.positions()
+    eng.step(1)
+    pg = eng.particles()
+    eng.from_device()
+    sq = dict(scene)
+    sq["x"] = xq
+    orc = O.Oracle(sq, pots)
+    assert orc.step(1, 4) == 0
+    po = orc.particles()
+    err = force_errors(pg["f"], po["f"])
+    nd = np.abs(pg["f"][:, 3] - po["f"][:, 3]).max() / float(np.max(po["f"][:, 3]))
+    ds = np.abs(pg["species"] - po["species"]).max()
+    print("dpd lists precision %d: force err %.3e density %.3e species %.3e" % (precision, err, nd, ds))
+    assert err <= tol
+    assert nd <= tol
+    assert ds <= (1e-12 if precision == 1 else 1e-6)

@@ -164,6 +164,8 @@ template <typename real> int run_advance(DeviceEnsemble *E, const KP<real> &P, i
 // stencil, run when the step rebuilds) and the list pair kernel.
 template <typename real> int run_list_build(DeviceEnsemble *E, const KP<real> &P);
 template <typename real> int run_list_force(DeviceEnsemble *E, const KP<real> &P);
+// The flux-only list pass of a flux sub-step (pending flux, no forces).
+template <typename real> int run_list_flux(DeviceEnsemble *E, const KP<real> &P);
 // Fluxes_integrate over every particle of buffer A (the flux sub-steps).
 template <typename real> int run_flux_integrate(DeviceEnsemble *E, const KP<real> &P);
 template <typename real> int run_energy(DeviceEnsemble *E, const KP<real> &P);

@@ -153,6 +153,12 @@ template <typename real> static int enqueue_step(DeviceEnsemble *E, bool energy,
         if (ev) TFG_HIP(hipEventRecord(ev[1], E->stream));
         if ((rc = run_list_build<real>(E, Pe)) != TFG_OK) return rc;
         if (ev) TFG_HIP(hipEventRecord(ev[2], E->stream));
+        // flux sub-steps (tf_engine_advance.cpp:313-323): the flux-only list
+        // pass, then Fluxes_integrate, nr_fluxsteps - 1 times
+        for (int k = 0; k < E->nr_fluxsteps - 1; k++) {
+            if (E->has_flux && (rc = run_list_flux<real>(E, P)) != TFG_OK) return rc;
+            if ((rc = run_flux_integrate<real>(E, P)) != TFG_OK) return rc;
+        }
         if ((rc = run_list_force<real>(E, Pe)) != TFG_OK) return rc;
         if (ev) TFG_HIP(hipEventRecord(ev[3], E->stream));
         if ((rc = run_advance<real>(E, Pe, 1)) != TFG_OK) return rc;
@@ -500,14 +506,14 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     E->max_sys_blocks = (max_np + energy_tile() - 1) / energy_tile();
     int max_nt = 1;
     for (const SysDesc &d : sys) max_nt = std::max(max_nt, d.ntypes);
-    // Lists carry tables only (no DPD, flux or radius transforms), with every
+    // Lists carry tables, DPD and flux (no radius transforms), with every
     // system's tables in LDS, and slots below 2^25.
-    E->list_mode = list_ok && !E->no_alloc && !E->has_dpd && !E->has_flux && !E->has_ext && np + 1 < (1 << kListSlotBits) &&
-                   static_cast<size_t>(max_coef) * sizeof(real) <= 16384 && max_coef > 0;
+    E->list_mode = list_ok && !E->no_alloc && !E->has_ext && np + 1 < (1 << kListSlotBits) &&
+                   static_cast<size_t>(max_coef) * sizeof(real) <= 16384;
     E->lcap = E->list_mode ? lcap : 0;
     int rc = nonbond_setup<real>(E, max_nt, max_coef);
     if (rc != TFG_OK) return rc;
-    if (E->list_mode && E->coef_lds_bytes == 0) E->list_mode = false;
+    if (E->list_mode && max_coef > 0 && E->coef_lds_bytes == 0) E->list_mode = false;
     std::vector<int4> lblocks;
     if (E->list_mode)
         for (int s = 0; s < nsys; s++)

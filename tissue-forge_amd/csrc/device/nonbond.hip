@@ -107,6 +107,16 @@ __device__ inline double dist2(double4 a, double4 b) {
 __device__ inline float fma_of(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ inline double fma_of(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Pending-flux accumulators held in registers: element idx (< S) added to
+// through compile-time indices (a runtime index would place the array in
+// scratch memory).  A particle's own species are read from memory where a
+// flux term needs them, for the same reason.
+template <typename real> __device__ inline void species_add(real (&a)[kMaxSpecies], int idx, int S, real v) {
+#pragma unroll
+    for (int k = 0; k < kMaxSpecies; k++)
+        if (k < S) a[k] += idx == k ? v : real(0);
+}
+
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -402,14 +412,11 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                 v4 vi4;
                 if (DPD) vi4 = P.A.vel[si];
                 const int id_i = mi.y - sd.p_off;
-                real spi[kMaxSpecies];
-                if (FLUX)
-                    for (int k = 0; k < S; k++) spi[k] = P.A.sp[(size_t)k * P.np + si];
+                real qacc[kMaxSpecies];
+#pragma unroll
+                for (int k = 0; k < kMaxSpecies; k++) qacc[k] = real(0);
 
                 real fx = 0, fy = 0, fz = 0, nd = 0, ep = 0, vir = 0;
-                real qacc[kMaxSpecies];
-                if (FLUX)
-                    for (int k = 0; k < S; k++) qacc[k] = 0;
                 int self_k = -1;   // staged index of the i particle itself in the current chunk
                 int lbase = 0;     // BUILD: list entries of this i particle so far
                 if (BUILD) s_self[lane] = -1;
@@ -560,8 +567,8 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                             // Role of this particle in the entry (tf_flux_eval.h:64-67):
                             // the type_a side; between equal types the lower id.
                             const bool i_is_a = (ti == fd.type_a) && (ti != tj || id_i < id_j);
-                            const real sa = i_is_a ? spi[fd.index_a] : s_sp[k * S + fd.index_a];
-                            const real sb = i_is_a ? s_sp[k * S + fd.index_b] : spi[fd.index_b];
+                            const real sa = i_is_a ? P.A.sp[(size_t)fd.index_a * P.np + si] : s_sp[k * S + fd.index_a];
+                            const real sb = i_is_a ? s_sp[k * S + fd.index_b] : P.A.sp[(size_t)fd.index_b * P.np + si];
                             real qv;
                             if (fd.kind == FLUX_FICK) {
                                 qv = fd.coef * (sa - sb);
@@ -574,9 +581,9 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                             }
                             qv *= term;
                             if (i_is_a)
-                                qacc[fd.index_a] += -qv - fd.half_decay * sa;
+                                species_add(qacc, fd.index_a, S, -qv - fd.half_decay * sa);
                             else
-                                qacc[fd.index_b] += qv - fd.half_decay * sb;
+                                species_add(qacc, fd.index_b, S, qv - fd.half_decay * sb);
                         }
                     }
                     if constexpr (!EXT) {
@@ -929,12 +936,14 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     s_red[4 * kWave + lane] = ep;
                     s_red[5 * kWave + lane] = vir;
                     if (FLUX)
-                        for (int k = 0; k < S; k++) s_red[(6 + k) * kWave + lane] = qacc[k];
+#pragma unroll
+                        for (int k = 0; k < kMaxSpecies; k++)
+                            if (k < S) s_red[(6 + k) * kWave + lane] = qacc[k];
                     wave_sync();
                     if (active && sl == 0) {
                         fx = fy = fz = nd = ep = vir = real(0);
-                        if (FLUX)
-                            for (int k = 0; k < S; k++) qacc[k] = real(0);
+#pragma unroll
+                        for (int k = 0; k < kMaxSpecies; k++) qacc[k] = real(0);
                         for (int s = 0; s < slices; s++) {
                             fx += s_red[0 * kWave + lane + s];
                             fy += s_red[1 * kWave + lane + s];
@@ -943,7 +952,9 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                             ep += s_red[4 * kWave + lane + s];
                             vir += s_red[5 * kWave + lane + s];
                             if (FLUX)
-                                for (int k = 0; k < S; k++) qacc[k] += s_red[(6 + k) * kWave + lane + s];
+#pragma unroll
+                                for (int k = 0; k < kMaxSpecies; k++)
+                                    if (k < S) qacc[k] += s_red[(6 + k) * kWave + lane + s];
                         }
                     }
                     wave_sync();
@@ -956,7 +967,9 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                     fo.w = nd + real(sd.w_self);
                     P.force[si] = fo;
                     if (FLUX)
-                        for (int k = 0; k < S; k++) P.A.q[(size_t)k * P.np + si] += qacc[k];
+#pragma unroll
+                        for (int k = 0; k < kMaxSpecies; k++)
+                            if (k < S) P.A.q[(size_t)k * P.np + si] += qacc[k];
                     if (ENERGY) {
                         P.e_nb[si] = 0.5 * static_cast<double>(ep);
                         P.e_vir[si] = 0.5 * static_cast<double>(vir);
@@ -1098,8 +1111,12 @@ __device__ inline double list_guard_r2(const KP<double> &, const SysDesc &, int,
 
 // Pair forces from the neighbour lists: one lane per particle over its
 // entries in list order (the build's candidate order), so each particle's sum
-// has a fixed order.  Tables only (no DPD, flux or radius transforms: those
-// systems use the cell kernel).  A workgroup covers up to kNbThreads
+// has a fixed order.  Tables, and with DPD / FLUX set DPD pairs (the partner's
+// velocity and id gathered per entry) and species flux (its species); no
+// radius transforms (those systems use the cell kernel).  FORCES = false is
+// the flux-only pass of the flux sub-steps (the role of
+// runner_dopair_fluxonly, tf_runner_dopair.cpp:334-412): pending flux only,
+// no forces, density or energies.  A workgroup covers up to kNbThreads
 // particles of one system and caches its pair descriptors, tables and the
 // the 125 build-cell shifts (in FP64) in LDS.  The pair test, the FP32 cutoff
 // guard (band pairs decided in FP64 from the stored positions), the number
@@ -1114,12 +1131,15 @@ __device__ inline double list_guard_r2(const KP<double> &, const SysDesc &, int,
 // partner positions gathered together, then evaluated in list order).  The
 // build pads every list to whole batches with the dummy slot np, so each lane
 // runs its own number of batches with no per-entry test of the list end.
-template <typename real, bool ENERGY>
+template <typename real, bool ENERGY, bool DPD = false, bool FLUX = false, bool FORCES = true>
 __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     using v4 = typename V<real>::v4;
     constexpr bool kGuarded = std::is_same<real, float>::value;
     constexpr int kCol = type_column<v4>(1);   // bytes per descriptor column
+    constexpr bool kIds = DPD || FLUX;          // the noise and the flux roles compare ids
     extern __shared__ __align__(16) unsigned char s_dyn[];
+    __shared__ FluxDesc<real> s_flux[FLUX ? kMaxFluxEntries : 1];
+    __shared__ int2 s_fmap[FLUX ? kMaxTypes * kMaxTypes : 1];
     if (P.err[0] != 0) return;   // halted (nothing writes the word during this kernel)
     // Workgroups go to the 8 XCDs in turn: the workgroups of one XCD take
     // consecutive particle blocks (the grid is a multiple of 8), so that the
@@ -1138,6 +1158,11 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
         s_sh[c] = make_double2((c / 25 - 2) * sd.h[0], (c / 5 % 5 - 2) * sd.h[1]);
         s_sh[kListShiftZ / 16 + c] = make_double2((c % 5 - 2) * sd.h[2], 0.0);
     }
+    if (FLUX) {
+        for (int t = threadIdx.x; t < nt * nt; t += kNbThreads)
+            s_fmap[t] = make_int2(P.fluxmap[2 * (sd.flux_off + t)], P.fluxmap[2 * (sd.flux_off + t) + 1]);
+        for (int t = threadIdx.x; t < sd.nflux; t += kNbThreads) s_flux[t] = P.flux[sd.flux_first + t];
+    }
     __syncthreads();
 
     const bool valid = static_cast<int>(threadIdx.x) < lb.z;
@@ -1152,11 +1177,21 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
 
     const v4 pi = P.xb[i];
     const double pxd = static_cast<double>(pi.x), pyd = static_cast<double>(pi.y), pzd = static_cast<double>(pi.z);
-    const int ti = P.A.meta[i].x & 0xffff;
+    const int2 mi = *reinterpret_cast<const int2 *>(&P.A.meta[i]);
+    const int ti = mi.x & 0xffff;
     const int ti_row = ti * nt * kCol;
     const int n = valid ? min(P.lcount[i], P.lcap) : 0;
     const int nbat = (n + kListBatch - 1) & ~(kListBatch - 1);   // the build padded the list to here
     real fx = 0, fy = 0, fz = 0, nd = 0, ep = 0, vir = 0;
+    const int S = P.S;
+    v4 vi4;
+    if (DPD) vi4 = P.A.vel[i];
+    real qacc[kMaxSpecies];
+#pragma unroll
+    for (int k = 0; k < kMaxSpecies; k++) qacc[k] = real(0);
+    const long long tstep = sd.time0 + *P.step + P.step_ahead;
+    const real dpd_delta = real(sd.inv_sqrt_dt);
+    (void)tstep; (void)dpd_delta; (void)S;
     const int *lp = P.list + list_index(0, i, P.lcap);   // entry k at lp[64 k]
     // The entries of the next batch are loaded while this batch computes (they
     // stream from HBM); the partner positions of a batch are gathered at its
@@ -1171,10 +1206,18 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     for (int k0 = 0; k0 < nbat; k0 += kListBatch) {
         unsigned ev[kListBatch];
         v4 pv[kListBatch];
+        v4 vv[DPD ? kListBatch : 1];
+        int idv[kIds ? kListBatch : 1];
 #pragma unroll
         for (int u = 0; u < kListBatch; u++) ev[u] = en[u];
 #pragma unroll
-        for (int u = 0; u < kListBatch; u++) pv[u] = P.xb[ev[u] & kListSlotMask];
+        for (int u = 0; u < kListBatch; u++) {
+            const int j = static_cast<int>(ev[u] & kListSlotMask);
+            pv[u] = P.xb[j];
+            // the dummy slot np pads the lists: its velocity and id are never used
+            if (DPD) vv[u] = P.A.vel[min(j, P.np - 1)];
+            if (kIds) idv[u] = P.A.meta[min(j, P.np - 1)].y;
+        }
         // memory counters retire in issue order: the gathers go first, so that
         // waiting for them never waits for the next batch's entries
         __builtin_amdgcn_sched_barrier(0);
@@ -1212,7 +1255,60 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
                     in = ok && r2d <= bd * bd;
                 }
             }
+            if (FLUX && ok) {
+                // flux_eval_ex (tf_flux_eval.h:48-103), as in the cell kernel
+                const int tj = real_bits_as_int(pj.w) / kCol;
+                const int2 fm = s_fmap[ti * nt + tj];
+                const int id_i = mi.y, id_j = idv[u];
+                for (int f = 0; f < fm.y; f++) {
+                    const FluxDesc<real> &fd = s_flux[fm.x + f];
+                    if (r_raw > fd.cutoff) continue;
+                    real term = real(1) - r_raw / fd.cutoff;
+                    term = term * term;
+                    // the type_a side; between equal types the lower id (tf_flux_eval.h:64-67)
+                    const bool i_is_a = (ti == fd.type_a) && (ti != tj || id_i < id_j);
+                    const int sa_of = i_is_a ? i : j, sb_of = i_is_a ? j : i;
+                    const real sa = P.A.sp[(size_t)fd.index_a * P.np + sa_of];
+                    const real sb = P.A.sp[(size_t)fd.index_b * P.np + sb_of];
+                    real qv;
+                    if (fd.kind == FLUX_FICK) {
+                        qv = fd.coef * (sa - sb);
+                    } else if (fd.kind == FLUX_SECRETE) {
+                        qv = fd.coef * (sa - fd.target);
+                        qv = qv > real(0) ? qv : real(0);
+                    } else {
+                        qv = fd.coef * (fd.target - sb) * sa;
+                        qv = qv > real(0) ? qv : real(0);
+                    }
+                    qv *= term;
+                    if (i_is_a)
+                        species_add(qacc, fd.index_a, S, -qv - fd.half_decay * sa);
+                    else
+                        species_add(qacc, fd.index_b, S, qv - fd.half_decay * sb);
+                }
+            }
+            if constexpr (!FORCES) continue;
             nd += ok ? w0 + r2 * (w2 + r_raw * w3) : real(0);
+            if (DPD && (real_bits_as_int(q1.w) & kMetaDpd)) {
+                // dpd_eval (tf_dpd_eval.h:32-76), as in the cell kernel
+                const real ro = r2 < eps2 ? eps_of<real>() : r_raw;
+                const real r = r_raw >= q0.x ? r_raw : q0.x;
+                const real ex = dx * ir, ey = dy * ir, ez = dz * ir;
+                const v4 vj = vv[u];
+                const real evel = ex * (vi4.x - vj.x) + ey * (vi4.y - vj.y) + ez * (vi4.z - vj.z);
+                const real om = r < real(0) ? real(1) : (real(1) - r * q1.y);
+                real xi = real(1);
+                if (sd.dpd_noise == DPD_NOISE_PHILOX)
+                    xi = dpd_xi<real>(static_cast<uint32_t>(mi.y - sd.p_off), static_cast<uint32_t>(idv[u] - sd.p_off),
+                                      static_cast<uint64_t>(tstep), sd.seed_lo, sd.seed_hi);
+                real f = q0.z * om - q0.w * om * om * evel + q1.x * om * dpd_delta * xi;
+                f = in ? f : real(0);
+                fx += f * ex;
+                fy += f * ey;
+                fz += f * ez;
+                if (ENERGY) vir += f * ro;
+                continue;
+            }
             const real r = r_raw > q0.x ? r_raw : q0.x;   // the table's left end (a select: no NaN is possible here)
             const int ind = static_cast<int>(clamp_real(q0.z + r * (q0.w + r * q1.x), real(1), q1.y));
             const int tb = real_bits_as_int(q1.z);
@@ -1236,29 +1332,54 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
         }
     }
     if (valid) {
-        v4 fo;
-        fo.x = fx;
-        fo.y = fy;
-        fo.z = fz;
-        fo.w = nd + real(sd.w_self);
-        P.force[i] = fo;
-        if (ENERGY) {
-            P.e_nb[i] = 0.5 * static_cast<double>(ep);
-            P.e_vir[i] = 0.5 * static_cast<double>(vir);
+        if (FLUX)
+#pragma unroll
+            for (int k = 0; k < kMaxSpecies; k++)
+                if (k < S) P.A.q[(size_t)k * P.np + i] += qacc[k];
+        if (FORCES) {
+            v4 fo;
+            fo.x = fx;
+            fo.y = fy;
+            fo.z = fz;
+            fo.w = nd + real(sd.w_self);
+            P.force[i] = fo;
+            if (ENERGY) {
+                P.e_nb[i] = 0.5 * static_cast<double>(ep);
+                P.e_vir[i] = 0.5 * static_cast<double>(vir);
+            }
         }
     }
+}
+
+template <typename real, bool DPD, bool FLUX> static const void *list_fn(bool energy) {
+    return energy ? reinterpret_cast<const void *>(&k_list_force<real, true, DPD, FLUX>)
+                  : reinterpret_cast<const void *>(&k_list_force<real, false, DPD, FLUX>);
 }
 
 template <typename real> int run_list_force(DeviceEnsemble *E, const KP<real> &P) {
     KP<real> Q = P;
     Q.nb = E->nbl;
     void *args[] = {&Q};
-    const void *fn = P.energy ? reinterpret_cast<const void *>(&k_list_force<real, true>)
-                              : reinterpret_cast<const void *>(&k_list_force<real, false>);
+    const bool en = P.energy != 0;
+    const void *fn = E->has_dpd ? (E->has_flux ? list_fn<real, true, true>(en) : list_fn<real, true, false>(en))
+                                : (E->has_flux ? list_fn<real, false, true>(en) : list_fn<real, false, false>(en));
     const hipError_t e = hipLaunchKernel(fn, dim3(E->nlblocks), dim3(kNbThreads), args, E->nbl.total, E->stream);
     if (e != hipSuccess) return set_error(TFG_E_DEVICE, std::string("list pairs: ") + hipGetErrorString(e));
     return launch_error("list pairs");
 }
+
+template <typename real> int run_list_flux(DeviceEnsemble *E, const KP<real> &P) {
+    KP<real> Q = P;
+    Q.nb = E->nbl;
+    void *args[] = {&Q};
+    const void *fn = reinterpret_cast<const void *>(&k_list_force<real, false, false, true, false>);
+    const hipError_t e = hipLaunchKernel(fn, dim3(E->nlblocks), dim3(kNbThreads), args, E->nbl.total, E->stream);
+    if (e != hipSuccess) return set_error(TFG_E_DEVICE, std::string("list flux: ") + hipGetErrorString(e));
+    return launch_error("list flux");
+}
+
+template int run_list_flux<float>(DeviceEnsemble *, const KP<float> &);
+template int run_list_flux<double>(DeviceEnsemble *, const KP<double> &);
 
 template int run_list_force<float>(DeviceEnsemble *, const KP<float> &);
 template int run_list_force<double>(DeviceEnsemble *, const KP<double> &);
