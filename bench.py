@@ -266,8 +266,9 @@ LEGS = {
 }
 
 
-def run_leg(tf, W, O, name, warmup, threads, cpu=True):
+def run_leg(tf, W, O, name, warmup, threads, cpu=True, steps_override=0):
     build, prec, steps, (skin, capacity), dpd_flux, cw, cs, what = LEGS[name]
+    steps = steps_override or steps
     scene = build(W)
     pots = [W.make_potential(p) for p in scene["pots"]]
     n = scene["x"].shape[0]
@@ -364,6 +365,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-legs", action="store_true")
     ap.add_argument("--leg", choices=sorted(LEGS), help="time only this workload on the GPU")
+    ap.add_argument("--leg-steps", type=int, default=0, help="GPU steps of each leg (0: the workload's own count)")
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--ens-steps", type=int, default=50)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
@@ -386,7 +388,8 @@ def main():
     threads = args.cpu_threads or host_threads()
 
     if args.leg:
-        print(json.dumps({args.leg: run_leg(tf, W, O, args.leg, args.warmup, threads, cpu=False)}), flush=True)
+        print(json.dumps({args.leg: run_leg(tf, W, O, args.leg, args.warmup, threads, cpu=False,
+                                            steps_override=args.leg_steps)}), flush=True)
         return
 
     n = args.n
@@ -482,7 +485,8 @@ def main():
         out["cpu_baseline"] = base
         log("cpu baseline %s" % base)
     if not args.no_legs:
-        out["legs"] = {name: run_leg(tf, W, O, name, args.warmup, threads, cpu=not args.no_cpu) for name in LEGS}
+        out["legs"] = {name: run_leg(tf, W, O, name, args.warmup, threads, cpu=not args.no_cpu,
+                                     steps_override=args.leg_steps) for name in LEGS}
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble(tf, W, O, args.replicas, args.ens_steps, args.warmup, threads,
                                          cpu=not args.no_cpu)

@@ -3,10 +3,14 @@ Morse+LJ (4 types, density 1, cutoff 2.5, 50^3 cells) on the neighbour-list
 path with skin 0.4, the configuration bench.py times.
 
 The first step's pair pass (lists built from the input positions) against
-the FP64 oracle evaluated at the same positions, as the path holds them:
-mixed err_rms <= 1e-5 (force error bound of the FP32-mixed path), FP64 <=
-1e-10; number densities likewise; and the cell ids of the binned input equal
-the oracle's.  The oracle runs on up to 16 host threads (~6 s).
+the FP64 oracle evaluated at the same positions, as the path holds them,
+with err = max_i |dF_i| / rms|F| and rel_i = |dF_i| / max(|F_i|, 1e-3 rms|F|):
+  mixed  err <= 1e-5, rel_max <= 1e-4 (5.6e-5 measured: per-pair FP32
+         rounding on particles whose pair forces nearly cancel), the 99th
+         percentile of rel <= 1e-5, energy within 1e-6 relative;
+  FP64   err <= 1e-10, rel_max <= 1e-10, energy within 1e-12;
+number densities within the err bound, and the cell ids of the binned input
+equal the oracle's.  The oracle runs on up to 16 host threads (~6 s).
 """
 import math
 import os
@@ -22,8 +26,15 @@ def err_rms(fg, fo):
     return float(d.max() / math.sqrt(float(np.mean(np.sum(fo[:, :3] ** 2, axis=1)))))
 
 
-@pytest.mark.parametrize("precision,tol", [(0, 1e-5), (1, 1e-10)])
-def test_full_size_list_forces_match_oracle(tf, W, O, gpu, precision, tol):
+def rel_errors(fg, fo):
+    d = np.linalg.norm(fg[:, :3] - fo[:, :3], axis=1)
+    mag = np.linalg.norm(fo[:, :3], axis=1)
+    rel = d / np.maximum(mag, 1e-3 * math.sqrt(float(np.mean(mag ** 2))))
+    return float(rel.max()), float(np.percentile(rel, 99))
+
+
+@pytest.mark.parametrize("precision,tol,rel_tol,e_tol", [(0, 1e-5, 1e-4, 1e-6), (1, 1e-10, 1e-10, 1e-12)])
+def test_full_size_list_forces_match_oracle(tf, W, O, gpu, precision, tol, rel_tol, e_tol):
     scene = W.morse_lj(n=2_000_000)
     scene["verlet_skin"] = 0.4
     eng, pots = W.build_engine(scene)
@@ -32,17 +43,25 @@ def test_full_size_list_forces_match_oracle(tf, W, O, gpu, precision, tol):
     xq = eng.positions()
     eng.step(1)
     pg = eng.particles()
+    eg = eng.energies()
     eng.from_device()
     sq = dict(scene)
     sq["x"] = xq
     orc = O.Oracle(sq, pots)
     orc.force(max(1, min(16, os.cpu_count() or 1)))
     po = orc.particles()
+    eo = orc.energies()
     assert np.array_equal(po["cid"], eng_cid_at_input(tf, W, scene, precision))
     e = err_rms(pg["f"], po["f"])
+    rel_max, rel_p99 = rel_errors(pg["f"], po["f"])
+    de = abs(eg["epot"] - eo["epot"]) / abs(eo["epot"])
     nd = float(np.max(np.abs(pg["f"][:, 3] - po["f"][:, 3])) / np.max(po["f"][:, 3]))
-    print("precision", precision, "err_rms", e, "number density", nd)
+    print("precision", precision, "err_rms", e, "rel_max", rel_max, "rel_p99", rel_p99, "energy", de,
+          "number density", nd)
     assert e <= tol
+    assert rel_max <= rel_tol
+    assert rel_p99 <= tol
+    assert de <= e_tol
     assert nd <= tol
 
 

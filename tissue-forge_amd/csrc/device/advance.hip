@@ -564,16 +564,42 @@ __global__ __launch_bounds__(256) void k_energy_final(const double *part, int nb
         for (int k = 0; k < 5; k++) out[5 * blockIdx.x + k] = s[k][0];
 }
 
-// Species totals per system in FP64 (conservation check of the flux pass).
+// Species totals per system in FP64 (conservation check of the flux pass):
+// block (bx, s) sums particles [bx*tile, (bx+1)*tile) of system s for every
+// species into part[((s*nbx + bx)*kMaxSpecies + k], then one block per system
+// adds the partials in order.
 template <typename real>
-__global__ __launch_bounds__(256) void k_species_total(const SysDesc *sys, const real *sp, int np, int S, double *out,
+__global__ __launch_bounds__(256) void k_species_part(const SysDesc *sys, const real *sp, int np, int S, double *part,
+                                                      int nbx, const int *halt) {
+    if (*halt != 0) return;
+    __shared__ double s[256];
+    const SysDesc &sd = sys[blockIdx.y];
+    const int b0 = blockIdx.x * kEnergyTile;
+    for (int k = 0; k < S; k++) {
+        double a = 0;
+        for (int t = threadIdx.x; t < kEnergyTile; t += 256) {
+            const int li = b0 + t;
+            if (li >= sd.np) break;
+            a += static_cast<double>(sp[(size_t)k * np + sd.p_off + li]);
+        }
+        s[threadIdx.x] = a;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[((size_t)blockIdx.y * nbx + blockIdx.x) * kMaxSpecies + k] = s[0];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_species_final(const double *part, int nbx, int S, double *out,
                                                        const int *halt) {
     if (*halt != 0) return;
     __shared__ double s[256];
-    const SysDesc &sd = sys[blockIdx.x];
     for (int k = 0; k < S; k++) {
         double a = 0;
-        for (int i = sd.p_off + threadIdx.x; i < sd.p_off + sd.np; i += 256) a += static_cast<double>(sp[(size_t)k * np + i]);
+        for (int b = threadIdx.x; b < nbx; b += 256) a += part[((size_t)blockIdx.x * nbx + b) * kMaxSpecies + k];
         s[threadIdx.x] = a;
         __syncthreads();
         for (int off = 128; off > 0; off >>= 1) {
@@ -620,9 +646,12 @@ template <typename real> int run_energy(DeviceEnsemble *E, const KP<real> &P) {
     hipLaunchKernelGGL(k_energy_part, dim3(nbx, E->nsys), dim3(256), 0, E->stream, E->d_sys, E->e_nb, E->e_vir,
                        E->e_kin, E->e_bond, E->e_ang, E->e_part, nbx, E->d_err);
     hipLaunchKernelGGL(k_energy_final, dim3(E->nsys), dim3(256), 0, E->stream, E->e_part, nbx, E->d_energy, E->d_err);
-    if (E->S > 0)
-        hipLaunchKernelGGL(k_species_total<real>, dim3(E->nsys), dim3(256), 0, E->stream, E->d_sys, P.B.sp, E->np,
-                           E->S, E->d_spec_tot, E->d_err);
+    if (E->S > 0) {
+        hipLaunchKernelGGL(k_species_part<real>, dim3(nbx, E->nsys), dim3(256), 0, E->stream, E->d_sys, P.B.sp, E->np,
+                           E->S, E->sp_part, nbx, E->d_err);
+        hipLaunchKernelGGL(k_species_final, dim3(E->nsys), dim3(256), 0, E->stream, E->sp_part, nbx, E->S,
+                           E->d_spec_tot, E->d_err);
+    }
     return launch_error("energy");
 }
 

@@ -111,7 +111,7 @@ struct DeviceEnsemble {
     int2 *tmp = nullptr;
     double *e_part = nullptr, *e_nb = nullptr, *e_vir = nullptr, *e_kin = nullptr, *e_bond = nullptr,
            *e_ang = nullptr;
-    double *d_energy = nullptr, *d_spec_tot = nullptr;
+    double *d_energy = nullptr, *d_spec_tot = nullptr, *sp_part = nullptr;
     int *inc_off = nullptr, *inc = nullptr;
     int4 *d_bonds = nullptr, *d_angles = nullptr;
     long long *d_step = nullptr;

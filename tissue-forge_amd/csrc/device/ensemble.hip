@@ -629,6 +629,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     ALLOC(E->e_part, double, (size_t)5 * E->max_sys_blocks * nsys);
     ALLOC(E->d_energy, double, 5 * nsys);
     ALLOC(E->d_spec_tot, double, kMaxSpecies * nsys);
+    ALLOC(E->sp_part, double, (size_t)kMaxSpecies * E->max_sys_blocks * nsys);
     ALLOC(E->d_step, long long, 1);
     ALLOC(E->d_err, int, 2);
     ALLOC(E->inc_off, int, np + 1);
