@@ -4,7 +4,7 @@ Headline workload: the 2,000,000-particle Morse+LJ system (4 types, density
 1, cutoff 2.5, tabulated potentials), FP32-mixed path.  One step = binning,
 nonbonded pass, bonded pass, forward Euler with migration, all on the GPU
 with the particles resident in HBM.  By default the pair pass uses neighbour
-lists with a 0.4 skin (rebuilt on the device when a particle has moved more
+lists with a 0.5 skin (rebuilt on the device when a particle has moved more
 than half the skin; the pair set of every step is unchanged); the cell pass
 is timed beside it.
 
@@ -255,13 +255,13 @@ def accuracy(O, scene, pots, r, threads, precision):
 
 LEGS = {
     # name: (builder, precision, gpu steps, (skin, list capacity), dpd_flux, cpu warm-up, cpu steps,
-    #        description).  The chains' random walks pack up to ~270 partners within
-    #        cutoff + skin, past the default capacity of 256.
+    #        description).  The chains' random walks pack up to ~300 partners within
+    #        cutoff + skin, past the default capacity.
     "cell_sorting_8k": (lambda W: W.cell_sorting(), 1, 1000, (0.4, 0), False, 10, 1000,
                         "cell sorting, 8,000 particles, Morse, overdamped, FP64"),
     "dpd_droplets_250k": (lambda W: W.dpd_droplets(), 0, 500, (0.4, 0), True, 10, 20,
                           "DPD droplets, 250,000 particles, Fick flux, FP32-mixed"),
-    "bonded_chains_1M": (lambda W: W.bonded_chains(), 0, 200, (0.4, 384), False, 10, 20,
+    "bonded_chains_1M": (lambda W: W.bonded_chains(), 0, 200, (0.6, 448), False, 10, 20,
                          "bonded chains, 1,000,000 particles, LJ + bonds + angles, FP32-mixed"),
 }
 
@@ -280,6 +280,15 @@ def run_leg(tf, W, O, name, warmup, threads, cpu=True, steps_override=0):
            "phases_ms": {k: r["kernels"][k] for k in PHASES}, "neighbour_lists": r["lists"],
            "roofline": roofline(r, n, prec, name, dpd_flux)}
     log("leg %s: %.4g particle-steps/s, phases %s" % (name, out["value"], out["phases_ms"]))
+    if dpd_flux:
+        # the same steps with three flux sub-steps per step (two flux-only
+        # passes and flux integrations more per step than the line above)
+        sub = dict(scene)
+        sub["nr_fluxsteps"] = 3
+        r3 = time_gpu(tf, W, sub, pots, prec, steps, warmup, skin, capacity)
+        out["flux_substeps_3"] = {"ms_per_step": r3["wall_graph_s"] / steps * 1e3,
+                                  "ms_per_extra_substep": (r3["wall_graph_s"] - r["wall_graph_s"]) / steps * 1e3 / 2}
+        log("leg %s with 3 flux sub-steps: %s" % (name, out["flux_substeps_3"]))
     if cpu:
         out["accuracy"], _ = accuracy(O, scene, pots, r, threads, prec)
         out["cpu_baseline"] = cpu_baseline(O, scene, pots, threads, cw, cs, "the same system")
@@ -369,7 +378,7 @@ def main():
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--ens-steps", type=int, default=50)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
-    ap.add_argument("--skin", type=float, default=0.4, help="neighbour-list skin (0: cell pass only)")
+    ap.add_argument("--skin", type=float, default=0.5, help="neighbour-list skin (0: cell pass only)")
     ap.add_argument("--no-cell", action="store_true", help="skip the cell-pass comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
