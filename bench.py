@@ -82,20 +82,23 @@ def traffic_bytes(kernel, workload):
 
 
 def counter_metrics(kernel, workload):
-    """VALU busy, LDS busy and LDS bank-conflict fractions of a kernel, from
-    the committed counter summary (its "derived" line); {} without it."""
-    path = profile_path(kernel, workload, "pmc")
-    out = {}
-    if not os.path.exists(path):
-        return out
-    with open(path) as f:
-        for line in f:
-            if line.startswith("derived %s:" % kernel):
-                for item in line.split(":", 1)[1].split():
-                    k, v = item.split("=")
-                    out[k] = float(v)
-    if out:
-        out["source"] = os.path.relpath(path, ROOT)
+    """VALU busy, LDS busy and LDS bank-conflict fractions of a kernel, and
+    its L1 (TCP) accesses per CU-cycle where measured, from the committed
+    counter summaries (their "derived" lines); {} without them."""
+    out, sources = {}, []
+    for kind in ("pmc", "l1"):
+        path = profile_path(kernel, workload, kind)
+        if not os.path.exists(path):
+            continue
+        with open(path) as f:
+            for line in f:
+                if line.startswith("derived %s:" % kernel):
+                    for item in line.split(":", 1)[1].split():
+                        k, v = item.split("=")
+                        out[k] = float(v)
+                    sources.append(os.path.relpath(path, ROOT))
+    if sources:
+        out["source"] = ", ".join(sources)
     return out
 
 
