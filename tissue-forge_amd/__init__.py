@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtfg.so")
+# TFG_LIB names another build of the library (the sanitizer builds of build.py)
+LIB_PATH = os.environ.get("TFG_LIB") or os.path.join(_HERE, "libtfg.so")
 
 NEWTONIAN = 0
 OVERDAMPED = 1

@@ -1,6 +1,16 @@
 """Builds libtfg.so (HIP device module + C ABI + host engine) in-tree for gfx950.
 
-Usage: python tissue-forge_amd/build.py
+Usage: python tissue-forge_amd/build.py [--sanitize address|undefined]
+
+--sanitize builds a host-instrumented copy (device code unchanged) into
+tissue-forge_amd/_build_<name>/libtfg.so, for the CPU test suite:
+
+  TFG_LIB=tissue-forge_amd/_build_address/libtfg.so ASAN_OPTIONS=detect_leaks=0 \
+  LD_PRELOAD=$(/opt/rocm/llvm/bin/clang -print-file-name=libclang_rt.asan-x86_64.so) \
+  python -m pytest tests -m "not gpu"
+
+(the sanitizer runtime is clang's, preloaded because Python itself is not
+instrumented; with undefined, libclang_rt.ubsan_standalone-x86_64.so).
 """
 import os
 import subprocess
@@ -47,14 +57,24 @@ def _run(cmd):
     return res.returncode, msgs
 
 
-def build(verbose=False, extra_flags=()):
-    """Compiles every source (in parallel) and links tissue-forge_amd/libtfg.so;
-    raises on failure."""
-    os.makedirs(OBJDIR, exist_ok=True)
+def build(verbose=False, extra_flags=(), sanitize=None):
+    """Compiles every source (in parallel) and links tissue-forge_amd/libtfg.so
+    (with sanitize, a host-instrumented copy in _build_<sanitize>/); raises on
+    failure."""
+    objdir, lib, link_flags = OBJDIR, LIB, []
+    flags = FLAGS + list(extra_flags)
+    if sanitize:
+        objdir = os.path.join(HERE, "_build_" + sanitize)
+        lib = os.path.join(objdir, "libtfg.so")
+        # host code only: the GPU code of this pool is never built instrumented
+        san = ["-Xarch_host", "-fsanitize=" + sanitize]
+        flags = [f if f != "-O3" else "-O1" for f in flags] + ["-g"] + san + ["-Xarch_host", "-fno-omit-frame-pointer"]
+        link_flags = san
+    os.makedirs(objdir, exist_ok=True)
     jobs = []
     for src in SOURCES:
-        obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
-        jobs.append((src, obj, [HIPCC] + FLAGS + list(extra_flags) + ["-c", src, "-o", obj]))
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        jobs.append((src, obj, [HIPCC] + flags + ["-c", src, "-o", obj]))
     if verbose:
         for _, _, cmd in jobs:
             print(" ".join(cmd), flush=True)
@@ -65,14 +85,20 @@ def build(verbose=False, extra_flags=()):
             raise RuntimeError("hipcc failed on %s:\n%s" % (src, "\n".join(msgs)))
         if verbose and msgs:
             print("\n".join(msgs))
-    tmp = LIB + ".tmp"
-    rc, msgs = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + [o for _, o, _ in jobs] + ["-o", tmp])
+    tmp = lib + ".tmp"
+    rc, msgs = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + link_flags + [o for _, o, _ in jobs] +
+                    ["-o", tmp])
     if rc != 0:
         raise RuntimeError("link failed:\n" + "\n".join(msgs))
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    san = None
+    if "--sanitize" in sys.argv:
+        san = sys.argv[sys.argv.index("--sanitize") + 1]
+        if san not in ("address", "undefined"):
+            sys.exit("--sanitize takes address or undefined")
+    print(build(verbose=True, sanitize=san))
     sys.exit(0)
