@@ -109,6 +109,47 @@ def test_list_graph_equals_eager_and_rebuilds(tf, W, gpu):
     assert np.abs(a[0]["x"] - c[0]["x"]).max() <= 1e-9
 
 
+def test_list_reads_between_steps(tf, W, gpu):
+    """A lean list-mode step updates its state in place; reads of the state
+    between steps (downloads, a force pass, a checkpoint) see the latest
+    update, and the run continues as if uninterrupted.  FP64: a read changes
+    no bit of the state; a force pass reorders the particles and rebuilds the
+    lists (sums in another order), so the continuation agrees to 1e-9."""
+    def engine():
+        scene = W.morse_lj(n=40_000)
+        scene["verlet_skin"] = 0.3
+        eng, _ = W.build_engine(scene)
+        eng.to_device(1)
+        assert eng.verlet_state()[0]
+        return eng
+    a = engine()
+    a.step(7)
+    pa7 = a.particles()
+    a.step(13)
+    pa = a.particles()
+    b = engine()
+    b.step(7)
+    pb7 = b.particles()
+    doc = b.to_json()                     # a checkpoint of the device state
+    pb7_again = b.particles()
+    b.step(5)
+    b.force()                             # rebins, and the next step rebuilds
+    b.step(8)
+    pb = b.particles()
+    c = tf.Engine.from_json(doc)          # the checkpoint, continued on the device
+    c.to_device(1)
+    c.step(13)
+    pc = c.particles()
+    for e in (a, b, c):
+        e.from_device()
+    for k in ("x", "v", "cid"):
+        assert np.array_equal(pa7[k], pb7[k]) and np.array_equal(pb7[k], pb7_again[k]), k
+    assert np.array_equal(pa["cid"], pb["cid"]) and np.array_equal(pa["cid"], pc["cid"])
+    assert np.abs(pa["x"] - pb["x"]).max() <= 1e-9
+    assert np.abs(pa["x"] - pc["x"]).max() <= 1e-9
+    assert np.abs(pa["v"] - pb["v"]).max() <= 1e-7 * np.abs(pa["v"]).max()
+
+
 def test_list_cutoff_guard(tf, W, O, gpu):
     """The FP32 cutoff guard decides band pairs in FP64 on the list path too."""
     import test_cutoff_guard as G

@@ -194,6 +194,9 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
         }
         return;
     }
+    // Lean list-mode steps update A in place; A is then newer than B.
+    const bool inplace = LEAN && !RK4 && P.inplace != 0 && update == 1;
+    if (inplace && blockIdx.x == 0 && threadIdx.x == 0) P.lflags[3] = 1;
     for (int base = wstart; base < P.np; base += stride) {
         const int i = base + lane;
         const bool valid = i < P.np;
@@ -450,10 +453,16 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                     P.B.vk[i] = v4_of<real>(vkn[0], vkn[1], vkn[2]);
                 }
             }
-            P.B.pos[i] = p;
-            P.B.vel[i] = v;
-            P.B.meta[i] = make_int4(info, id, newc, 0);
-            P.B.imass[i] = P.A.imass[i];
+            if (inplace) {   // each particle reads only its own slot of A
+                P.A.pos[i] = p;
+                P.A.vel[i] = v;
+                P.A.meta[i] = make_int4(info, id, newc, 0);
+            } else {
+                P.B.pos[i] = p;
+                P.B.vel[i] = v;
+                P.B.meta[i] = make_int4(info, id, newc, 0);
+                P.B.imass[i] = P.A.imass[i];
+            }
             if (!RK4 && P.list_mode && update == 1) {
                 // Position in the frame of the build cell (one FP64 rounding)
                 // and the displacement since the build: beyond skin / 2 the

@@ -193,16 +193,45 @@ __global__ __launch_bounds__(64) void k_cellorder(const int *start, const int2 *
     }
 }
 
+// Buffer B brought up to date from A after in-place updates (lflags[3]):
+// before a rebuilding step's binning (all reads of B follow), or always
+// (force, a force pass or a download).  Nothing to do while B is current.
+template <typename real>
+__global__ void k_sync(KP<real> P, int force) {
+    if (P.lflags[3] == 0) return;
+    if (!force && P.lflags[0] == 0) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.np; i += gridDim.x * blockDim.x) {
+        P.B.pos[i] = P.A.pos[i];
+        P.B.vel[i] = P.A.vel[i];
+        P.B.meta[i] = P.A.meta[i];
+        P.B.imass[i] = P.A.imass[i];
+    }
+}
+
+template <typename real> int run_sync(DeviceEnsemble *E, const KP<real> &P, bool force) {
+    if (!E->inplace) return TFG_OK;
+    hipLaunchKernelGGL(k_sync<real>, dim3(E->stream_grid()), dim3(E->shape.stream_threads), 0, E->stream, P,
+                       force ? 1 : 0);
+    return launch_error("sync");
+}
+
+template int run_sync<float>(DeviceEnsemble *, const KP<float> &, bool);
+template int run_sync<double>(DeviceEnsemble *, const KP<double> &, bool);
+
 template <typename real>
 __global__ void k_permute(KP<real> P) {
     using v4 = typename V<real>::v4;
     const int S = P.S;
     if (P.err[0] != 0) return;   // halted
     const bool any = !P.list_mode || P.lflags[0] != 0;
+    // in-place updates: A already holds the state of systems that keep their lists
+    const bool a_current = P.inplace && P.lflags[3] != 0;
+    if (a_current && !any) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.np; i += gridDim.x * blockDim.x) {
         // slot i belongs to the system of the particle buffer B holds there;
         // systems that keep their lists keep their order
         const bool rebuild = any && (!P.list_mode || P.sflags[P.nsys + P.cell_sys[P.B.meta[i].z]] != 0);
+        if (a_current && !rebuild) continue;
         const int src = rebuild ? P.perm[i] : i;
         P.A.pos[i] = P.B.pos[src];
         P.A.vel[i] = P.B.vel[src];
@@ -244,6 +273,8 @@ template <typename real> int run_bin(DeviceEnsemble *E, const KP<real> &P) {
     const int nb = (E->ncells + kScanTile - 1) / kScanTile;
     const int g = E->stream_grid(), t = E->shape.stream_threads;
     const int *gate = E->list_mode ? E->d_lflags : nullptr;
+    int rc;
+    if ((rc = run_sync<real>(E, P, false)) != TFG_OK) return rc;   // a rebuild bins from B
     if (E->list_mode)
         hipLaunchKernelGGL(k_count, dim3(g), dim3(t), 0, s, E->meta[1], E->cnt, E->np, gate, E->d_err);
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->ncells, gate, E->d_err);

@@ -184,6 +184,8 @@ template <typename real> struct KP {
     NbLayout nb;
     // Neighbour lists (null and 0 without them).
     int list_mode;
+    int inplace;                      // lean list-mode steps update buffer A in place (lflags[3] set once A
+                                      // is newer than B; k_sync copies A to B where B must be current)
     int lcap;                         // entries per particle
     int *list;                        // entry k of particle i at list[list_index(k, i, lcap)]
     int *lcount;                      // entries per particle at the build
@@ -191,7 +193,7 @@ template <typename real> struct KP {
     typename V<real>::v4 *xb;         // position in the frame of the build cell; w: type column bits
     typename V<real>::v4 *x0;         // stored position at the build
     int *cid0;                        // cell at the build
-    int *lflags;                      // [0] some system rebuilds in this step, [2] overflow
+    int *lflags;                      // [0] some system rebuilds in this step, [2] overflow, [3] A newer than B
     int *sflags;                      // per system: [s] requested, [nsys + s] rebuilds now, [2 nsys + s] rebuilds done
     int nsys;
 };

@@ -67,6 +67,7 @@ struct DeviceEnsemble {
     // transforms, sums, periodic images)
     bool has_dpd = false, has_flux = false, has_ext = false;
     bool lean_advance = false;   // no bonds, angles, active walls or species: the lean update kernel
+    bool inplace = false;        // list mode with the lean update: steps update buffer A in place
     int integrator = INTEG_FORWARD_EULER;   // shared by every system of the ensemble
     int nr_fluxsteps = 1;                   // shared as well
     int coef_lds_bytes = 0;   // table bytes cached in LDS by the pair kernel (0: read from global)
@@ -153,6 +154,7 @@ int ensemble_force_arena(Engine *e, int precision, void *arena, size_t cap, doub
 // Binning: scan of the cell counts, placement, per-cell order, permutation
 // of buffer B into buffer A.  Cell counts come from the previous advance.
 template <typename real> int run_bin(DeviceEnsemble *E, const KP<real> &P);
+template <typename real> int run_sync(DeviceEnsemble *E, const KP<real> &P, bool force);
 // Counts particles per cell from buffer B (after an upload).
 int run_count(DeviceEnsemble *E);
 template <typename real> int run_nonbond(DeviceEnsemble *E, const KP<real> &P);

@@ -72,6 +72,7 @@ template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy) {
     P.coef_lds = E->coef_lds_bytes > 0 ? 1 : 0;
     P.nb = E->nb;
     P.list_mode = E->list_mode ? 1 : 0;
+    P.inplace = E->inplace ? 1 : 0;
     P.lcap = E->lcap;
     P.list = E->d_list;
     P.lcount = E->d_lcount;
@@ -571,6 +572,11 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     E->lean_advance = deg[np] == 0 && E->S == 0;
     for (const SysDesc &d : E->sys)
         if (d.bc_active) E->lean_advance = false;
+    // List mode with the lean update: a step's update writes buffer A in
+    // place (each particle reads only itself), B is brought up to date only
+    // where it is read (a rebuilding step's binning, a force pass, a
+    // download), and steps that keep their lists skip the copy back to A.
+    E->inplace = E->list_mode && E->lean_advance;
     {
         std::vector<int> fillp(deg.begin(), deg.end() - 1);
         for (size_t b = 0; b < hb.size(); b++) {
@@ -721,6 +727,8 @@ int ensemble_destroy(DeviceEnsemble *E, bool download) {
 
 template <typename real> static int download_t(DeviceEnsemble *E, int only) {
     using v4 = typename V<real>::v4;
+    int rs;
+    if ((rs = run_sync<real>(E, make_params<real>(E, 0), true)) != TFG_OK) return rs;
     TFG_HIP(hipStreamSynchronize(E->stream));
     const int np = E->np, S = E->S;
     std::vector<v4> hpos(np), hvel(np), hf(np);
@@ -923,6 +931,7 @@ template <typename real> static int force_t(DeviceEnsemble *E) {
     P.step_ahead = 1;   // the evaluation belongs to the next step (engine_step increments time first)
     int rc;
     const bool timed = E->nev > 0;
+    if ((rc = run_sync<real>(E, P, true)) != TFG_OK) return rc;   // the binning reads B
     if (E->list_mode)
         hipLaunchKernelGGL(k_list_rebin, dim3(1), dim3(256), 0, E->stream, E->d_lflags, E->d_sflags, E->nsys);
     if (timed) TFG_HIP(hipEventRecord(E->ev[0], E->stream));
