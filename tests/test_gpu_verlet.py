@@ -109,15 +109,20 @@ def test_list_graph_equals_eager_and_rebuilds(tf, W, gpu):
     assert np.abs(a[0]["x"] - c[0]["x"]).max() <= 1e-9
 
 
-def test_list_reads_between_steps(tf, W, gpu):
-    """A lean list-mode step updates its state in place; reads of the state
-    between steps (downloads, a force pass, a checkpoint) see the latest
-    update, and the run continues as if uninterrupted.  FP64: a read changes
-    no bit of the state; a force pass reorders the particles and rebuilds the
-    lists (sums in another order), so the continuation agrees to 1e-9."""
+@pytest.mark.parametrize("name", ["morse_lj", "dpd_flux"])
+def test_list_reads_between_steps(tf, W, gpu, name):
+    """A list-mode step without bonds updates its state in place; reads of
+    the state between steps (downloads, a force pass, a checkpoint) see the
+    latest update, and the run continues as if uninterrupted.  FP64: a read
+    changes no bit of the state; a force pass reorders the particles and
+    rebuilds the lists (sums in another order), so the continuation agrees
+    to 1e-9."""
     def engine():
-        scene = W.morse_lj(n=40_000)
-        scene["verlet_skin"] = 0.3
+        if name == "morse_lj":
+            scene = W.morse_lj(n=40_000)
+            scene["verlet_skin"] = 0.3
+        else:
+            scene = _dpd_scene(W, n=6000, fluxsteps=2)
         eng, _ = W.build_engine(scene)
         eng.to_device(1)
         assert eng.verlet_state()[0]
@@ -142,12 +147,15 @@ def test_list_reads_between_steps(tf, W, gpu):
     pc = c.particles()
     for e in (a, b, c):
         e.from_device()
-    for k in ("x", "v", "cid"):
+    keys = ["x", "v", "cid"] + (["species", "species_q"] if "species" in pa else [])
+    for k in keys:
         assert np.array_equal(pa7[k], pb7[k]) and np.array_equal(pb7[k], pb7_again[k]), k
     assert np.array_equal(pa["cid"], pb["cid"]) and np.array_equal(pa["cid"], pc["cid"])
     assert np.abs(pa["x"] - pb["x"]).max() <= 1e-9
     assert np.abs(pa["x"] - pc["x"]).max() <= 1e-9
     assert np.abs(pa["v"] - pb["v"]).max() <= 1e-7 * np.abs(pa["v"]).max()
+    if "species" in pa:   # (a force pass adds its own pending flux, so not b's)
+        assert np.abs(pa["species"] - pc["species"]).max() <= 1e-9 * max(1.0, np.abs(pa["species"]).max())
 
 
 def test_list_cutoff_guard(tf, W, O, gpu):

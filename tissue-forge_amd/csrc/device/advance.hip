@@ -194,8 +194,9 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
         }
         return;
     }
-    // Lean list-mode steps update A in place; A is then newer than B.
-    const bool inplace = LEAN && !RK4 && P.inplace != 0 && update == 1;
+    // List-mode steps without bonds or angles update A in place; A is then
+    // newer than B.
+    const bool inplace = !RK4 && P.inplace != 0 && update == 1;
     if (inplace && blockIdx.x == 0 && threadIdx.x == 0) P.lflags[3] = 1;
     for (int base = wstart; base < P.np; base += stride) {
         const int i = base + lane;
@@ -500,8 +501,13 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                     s = s > real(0) ? s : real(0);
                     qq = real(0);
                 }
-                P.B.sp[o] = s;
-                P.B.q[o] = qq;
+                if (inplace) {
+                    P.A.sp[o] = s;
+                    P.A.q[o] = qq;
+                } else {
+                    P.B.sp[o] = s;
+                    P.B.q[o] = qq;
+                }
             }
         }
         if (P.list_mode) {
@@ -656,7 +662,9 @@ template <typename real> int run_energy(DeviceEnsemble *E, const KP<real> &P) {
                        E->e_kin, E->e_bond, E->e_ang, E->e_part, nbx, E->d_err);
     hipLaunchKernelGGL(k_energy_final, dim3(E->nsys), dim3(256), 0, E->stream, E->e_part, nbx, E->d_energy, E->d_err);
     if (E->S > 0) {
-        hipLaunchKernelGGL(k_species_part<real>, dim3(nbx, E->nsys), dim3(256), 0, E->stream, E->d_sys, P.B.sp, E->np,
+        // the species after the update: A when the update runs in place
+        hipLaunchKernelGGL(k_species_part<real>, dim3(nbx, E->nsys), dim3(256), 0, E->stream, E->d_sys,
+                           E->inplace ? P.A.sp : P.B.sp, E->np,
                            E->S, E->sp_part, nbx, E->d_err);
         hipLaunchKernelGGL(k_species_final, dim3(E->nsys), dim3(256), 0, E->stream, E->sp_part, nbx, E->S,
                            E->d_spec_tot, E->d_err);

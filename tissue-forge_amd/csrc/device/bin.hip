@@ -205,6 +205,10 @@ __global__ void k_sync(KP<real> P, int force) {
         P.B.vel[i] = P.A.vel[i];
         P.B.meta[i] = P.A.meta[i];
         P.B.imass[i] = P.A.imass[i];
+        for (int k = 0; k < P.S; k++) {
+            P.B.sp[(size_t)k * P.np + i] = P.A.sp[(size_t)k * P.np + i];
+            P.B.q[(size_t)k * P.np + i] = P.A.q[(size_t)k * P.np + i];
+        }
     }
 }
 
@@ -281,7 +285,9 @@ template <typename real> int run_bin(DeviceEnsemble *E, const KP<real> &P) {
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanBlock), 0, s, E->bsum, nb, gate, E->d_err);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->start, E->ncells, nb, gate, E->d_err);
     hipLaunchKernelGGL(k_scatter, dim3(g), dim3(t), 0, s, E->meta[1], E->start, E->fill, E->tmp, E->np, gate, E->d_err);
-    const int oc = std::min((E->ncells + 3) / 4, 65536);
+    // grid-stride over groups of four cells; a modest grid keeps the launch
+    // cheap in the list steps that skip it
+    const int oc = std::min((E->ncells + 3) / 4, 8192);
     hipLaunchKernelGGL(k_cellorder, dim3(oc), dim3(64), 0, s, E->start, E->tmp, E->perm, E->cnt, E->fill, E->ncells,
                        gate, E->d_err);
     hipLaunchKernelGGL(k_permute<real>, dim3(g), dim3(t), 0, s, P);

@@ -184,7 +184,7 @@ template <typename real> struct KP {
     NbLayout nb;
     // Neighbour lists (null and 0 without them).
     int list_mode;
-    int inplace;                      // lean list-mode steps update buffer A in place (lflags[3] set once A
+    int inplace;                      // list-mode steps without bonds update buffer A in place (lflags[3] set once A
                                       // is newer than B; k_sync copies A to B where B must be current)
     int lcap;                         // entries per particle
     int *list;                        // entry k of particle i at list[list_index(k, i, lcap)]

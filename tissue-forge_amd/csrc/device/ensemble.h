@@ -67,7 +67,7 @@ struct DeviceEnsemble {
     // transforms, sums, periodic images)
     bool has_dpd = false, has_flux = false, has_ext = false;
     bool lean_advance = false;   // no bonds, angles, active walls or species: the lean update kernel
-    bool inplace = false;        // list mode with the lean update: steps update buffer A in place
+    bool inplace = false;        // list mode without bonds or angles: steps update buffer A in place
     int integrator = INTEG_FORWARD_EULER;   // shared by every system of the ensemble
     int nr_fluxsteps = 1;                   // shared as well
     int coef_lds_bytes = 0;   // table bytes cached in LDS by the pair kernel (0: read from global)

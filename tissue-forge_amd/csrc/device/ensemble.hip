@@ -572,11 +572,11 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     E->lean_advance = deg[np] == 0 && E->S == 0;
     for (const SysDesc &d : E->sys)
         if (d.bc_active) E->lean_advance = false;
-    // List mode with the lean update: a step's update writes buffer A in
-    // place (each particle reads only itself), B is brought up to date only
-    // where it is read (a rebuilding step's binning, a force pass, a
+    // List mode without bonds or angles: a step's update writes buffer A in
+    // place (each particle reads only its own slot), B is brought up to date
+    // only where it is read (a rebuilding step's binning, a force pass, a
     // download), and steps that keep their lists skip the copy back to A.
-    E->inplace = E->list_mode && E->lean_advance;
+    E->inplace = E->list_mode && deg[np] == 0;
     {
         std::vector<int> fillp(deg.begin(), deg.end() - 1);
         for (size_t b = 0; b < hb.size(); b++) {
