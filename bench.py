@@ -18,8 +18,8 @@ one thread) and, under "legs", the other workloads: the 8,000-particle cell
 sorting system in FP64 (1,000 steps), the 250,000-particle DPD droplets
 with a Fick flux (500 steps) and the 1,000,000-particle bonded chains (200
 steps), each with its own phases, roofline, accuracy and CPU baseline, and
-the 512-replica ensemble, batched in one launch sequence against the same
-replicas stepped one after another.
+the 512-replica ensemble (neighbour lists, skin 0.3), batched in one launch
+sequence against the same replicas stepped one after another.
 
 --leg NAME runs only that workload's GPU timing (for profiler runs);
 --calibrate-copy runs only a plain device copy of known size (the reference
@@ -299,11 +299,14 @@ def run_leg(tf, W, O, name, warmup, threads, cpu=True, steps_override=0):
     return out
 
 
-def bench_ensemble(tf, W, O, nrep, steps, warmup, threads, cpu=True):
+def bench_ensemble(tf, W, O, nrep, steps, warmup, threads, cpu=True, skin=0.3):
     """The replica ensemble advanced as one system list (graph replay of the
     batched step) and the same replicas as separate engines, one after
-    another; the oracle steps a 16-replica subset one after another."""
+    another, both with neighbour lists of the given skin (0: the cell pass);
+    the oracle steps a 16-replica subset one after another."""
     scenes = W.ensemble(nrep=nrep)
+    for sc in scenes:
+        sc["verlet_skin"] = skin
     total = int(sum(sc["x"].shape[0] for sc in scenes))
     engines = [W.build_engine(sc)[0] for sc in scenes]
     ens = tf.Ensemble(engines, precision=tf.PREC_MIXED)
@@ -323,7 +326,7 @@ def bench_ensemble(tf, W, O, nrep, steps, warmup, threads, cpu=True):
     sequential = time.perf_counter() - t0
     for e in engines:
         e.close()
-    out = {"replicas": nrep, "particles": total, "steps": steps,
+    out = {"replicas": nrep, "particles": total, "steps": steps, "skin": skin,
            "batched_ms_per_step": batched / steps * 1e3, "sequential_ms_per_step": sequential / steps * 1e3,
            "batched_value": total * steps / batched, "sequential_value": total * steps / sequential,
            "speedup": sequential / batched, "autotuned_workgroups": {"pair": shape[0], "stream": shape[1]}}
@@ -379,6 +382,7 @@ def main():
     ap.add_argument("--leg", choices=sorted(LEGS), help="time only this workload on the GPU")
     ap.add_argument("--leg-steps", type=int, default=0, help="GPU steps of each leg (0: the workload's own count)")
     ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--ens-skin", type=float, default=0.3, help="the ensemble's list skin (0: cell pass)")
     ap.add_argument("--ens-steps", type=int, default=50)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     ap.add_argument("--skin", type=float, default=0.5, help="neighbour-list skin (0: cell pass only)")
@@ -501,7 +505,7 @@ def main():
                                      steps_override=args.leg_steps) for name in LEGS}
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble(tf, W, O, args.replicas, args.ens_steps, args.warmup, threads,
-                                         cpu=not args.no_cpu)
+                                         cpu=not args.no_cpu, skin=args.ens_skin)
         log("ensemble %s" % out["ensemble"])
     print(json.dumps(out), flush=True)
 

@@ -62,6 +62,45 @@ def test_list_trajectory_fp64(tf, W, O, gpu, name):
     assert rel(eg["ekin"], eo["ekin"]) <= 1e-9 or abs(eo["ekin"]) < 1e-12
 
 
+def test_list_small_periodic_boxes(tf, W, O, gpu):
+    """Boxes of three and four cells per axis with cutoff + skin above one
+    cell: the build stencil names some cells twice (under two shifts) and
+    keeps the near image.  Replicas of the batched-ensemble workload (the
+    smallest have rc = L / 3) against the oracle: ten FP64 steps, and the
+    mixed first-step forces at the FP32 positions."""
+    scenes = W.ensemble(nrep=40, nmin=500, nmax=3000)
+    picked = {}
+    for sc in sorted(scenes, key=lambda sc: len(sc["x"])):   # the smallest first
+        c = int(sc["cells"][0])
+        if c in (3, 4) and (sc["dim"][0] / c) < sc["cutoff"] + 0.3 and c not in picked:
+            picked[c] = sc
+    assert sorted(picked) == [3, 4]
+    for c, scene in sorted(picked.items()):
+        scene["verlet_skin"] = 0.3
+        eng, pots = W.build_engine(scene)
+        orc = O.Oracle(scene, pots)
+        eng.to_device(1)
+        assert eng.verlet_state()[0], scene["name"]
+        eng.step(10)
+        assert orc.step(10, 4) == 0
+        pg, po = eng.particles(), orc.particles()
+        eng.from_device()
+        print(scene["name"], "cells", c, "n", len(pg["x"]))
+        assert np.array_equal(pg["cid"], po["cid"])
+        assert np.abs(pg["x"] - po["x"]).max() <= 1e-9
+        eng, pots = W.build_engine(scene)
+        eng.to_device(0)
+        xq = eng.positions()
+        eng.step(1)
+        fg = eng.particles()["f"]
+        eng.from_device()
+        sq = dict(scene)
+        sq["x"] = xq
+        oq = O.Oracle(sq, pots)
+        oq.force(4)
+        assert force_errors(fg, oq.particles()["f"]) <= 1e-5
+
+
 @pytest.mark.parametrize("precision,tol", [(1, 1e-10), (0, 1e-5)])
 def test_list_forces(tf, W, O, gpu, precision, tol):
     """The forces of a list step (evaluated at the step's start) against the

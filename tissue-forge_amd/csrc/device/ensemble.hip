@@ -321,8 +321,12 @@ template <typename real> static int upload(DeviceEnsemble *E) {
             }
         d.nstencil = static_cast<int>(stencil.size() / 2) - d.stencil_off;
         // Neighbour lists: build stencil for cutoff + skin (at most two cells
-        // per axis, five cells or more along periodic axes), the build radius
-        // widened and the rebuild threshold narrowed by small margins.
+        // per axis), the build radius widened and the rebuild threshold
+        // narrowed by small margins.  Along a periodic axis the box must be
+        // wider than twice cutoff + skin, so that a pair has one image within
+        // reach; with fewer than 2 * span + 1 cells the stencil then names
+        // some cells twice, under two shifts, and the build keeps the near
+        // image (the other lies beyond reach).
         if (e.verlet_skin > 0) {
             const double rcs = e.geom.cutoff + e.verlet_skin;
             int sb[3];
@@ -330,7 +334,8 @@ template <typename real> static int upload(DeviceEnsemble *E) {
             const unsigned per = e.boundaries(fb);
             for (int k = 0; k < 3; k++) {
                 sb[k] = static_cast<int>(std::ceil(rcs / e.geom.h[k]));
-                if (sb[k] > 2 || ((per >> k & 1u) && e.geom.cdim[k] < 2 * sb[k] + 1)) list_ok = false;
+                if (sb[k] > 2 || ((per >> k & 1u) && e.geom.cdim[k] * e.geom.h[k] <= 2.0 * rcs * (1.0 + 1e-6)))
+                    list_ok = false;
             }
             d.span_z_b = sb[2];
             d.stencil_off_b = static_cast<int>(stencil.size() / 2);

@@ -1085,8 +1085,8 @@ template <typename real> int run_list_build(DeviceEnsemble *E, const KP<real> &P
 
 // FP64 squared distance of particles i and j from their stored positions and
 // current cells (the list kernel's cutoff guard): the cell offset is taken
-// to the nearest image on periodic axes (partners are at most two cells
-// apart and periodic list axes have at least five cells).
+// to the nearest image on periodic axes (partners are at most a few cells
+// apart, and a periodic list axis is wider than twice cutoff + skin).
 __device__ inline double list_guard_r2(const KP<float> &P, const SysDesc &sd, int i, int j) {
     const float4 a = P.A.pos[i], b = P.A.pos[j];
     int li[3], lj[3];
@@ -1101,7 +1101,14 @@ __device__ inline double list_guard_r2(const KP<float> &P, const SysDesc &sd, in
             if (2 * d > sd.cdim[k]) d -= sd.cdim[k];
             else if (2 * d < -sd.cdim[k]) d += sd.cdim[k];
         }
-        const double e = (static_cast<double>(av[k]) - static_cast<double>(bv[k])) - static_cast<double>(d) * sd.h[k];
+        double e = (static_cast<double>(av[k]) - static_cast<double>(bv[k])) - static_cast<double>(d) * sd.h[k];
+        if (sd.periodic & (1 << k)) {   // the nearest image (with four cells, d = +-2 names both)
+            const double len = sd.cdim[k] * sd.h[k];
+            if (2.0 * e > len || 2.0 * e < -len) {
+                d += e > 0.0 ? sd.cdim[k] : -sd.cdim[k];
+                e = (static_cast<double>(av[k]) - static_cast<double>(bv[k])) - static_cast<double>(d) * sd.h[k];
+            }
+        }
         r2 = __builtin_fma(e, e, r2);
     }
     return r2;
