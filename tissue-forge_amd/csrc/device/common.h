@@ -196,6 +196,7 @@ template <typename real> struct KP {
     int *lflags;                      // [0] some system rebuilds in this step, [2] overflow, [3] A newer than B
     int *sflags;                      // per system: [s] requested, [nsys + s] rebuilds now, [2 nsys + s] rebuilds done
     int nsys;
+    int lpp, lpp_log2;                // list pass: lanes per particle (a power of two) and its log2
 };
 
 __device__ inline float as_real_bits(int v, float) { return __int_as_float(v); }

@@ -78,6 +78,7 @@ struct DeviceEnsemble {
     // no_alloc: buffers from an arena, the null stream, no events, no lists
     bool no_alloc = false;
     int lcap = 0;
+    int lpp = 1, lpp_log2 = 0;  // list pass lanes per particle
     int nlblocks = 0;
     int4 *d_lblock = nullptr;
     int *d_list = nullptr, *d_lcount = nullptr, *d_cid0 = nullptr, *d_lflags = nullptr, *d_sflags = nullptr;

@@ -1085,8 +1085,8 @@ template <typename real> int run_list_build(DeviceEnsemble *E, const KP<real> &P
 
 // FP64 squared distance of particles i and j from their stored positions and
 // current cells (the list kernel's cutoff guard): the cell offset is taken
-// to the nearest image on periodic axes (partners are at most a few cells
-// apart, and a periodic list axis is wider than twice cutoff + skin).
+// to the nearest image on periodic axes (a periodic list axis is wider than
+// twice cutoff + skin, so a pair within reach has one such image).
 __device__ inline double list_guard_r2(const KP<float> &P, const SysDesc &sd, int i, int j) {
     const float4 a = P.A.pos[i], b = P.A.pos[j];
     int li[3], lj[3];
@@ -1100,15 +1100,13 @@ __device__ inline double list_guard_r2(const KP<float> &P, const SysDesc &sd, in
         if (sd.periodic & (1 << k)) {
             if (2 * d > sd.cdim[k]) d -= sd.cdim[k];
             else if (2 * d < -sd.cdim[k]) d += sd.cdim[k];
+            // half the box (an even cell count): the image on the side the
+            // positions in the cells lean to is the nearer one (on a tie both
+            // lie half a box away, beyond the cutoff)
+            if (2 * d == sd.cdim[k] && av[k] < bv[k]) d -= sd.cdim[k];
+            else if (2 * d == -sd.cdim[k] && av[k] > bv[k]) d += sd.cdim[k];
         }
-        double e = (static_cast<double>(av[k]) - static_cast<double>(bv[k])) - static_cast<double>(d) * sd.h[k];
-        if (sd.periodic & (1 << k)) {   // the nearest image (with four cells, d = +-2 names both)
-            const double len = sd.cdim[k] * sd.h[k];
-            if (2.0 * e > len || 2.0 * e < -len) {
-                d += e > 0.0 ? sd.cdim[k] : -sd.cdim[k];
-                e = (static_cast<double>(av[k]) - static_cast<double>(bv[k])) - static_cast<double>(d) * sd.h[k];
-            }
-        }
+        const double e = (static_cast<double>(av[k]) - static_cast<double>(bv[k])) - static_cast<double>(d) * sd.h[k];
         r2 = __builtin_fma(e, e, r2);
     }
     return r2;
@@ -1138,7 +1136,9 @@ __device__ inline double list_guard_r2(const KP<double> &, const SysDesc &, int,
 // partner positions gathered together, then evaluated in list order).  The
 // build pads every list to whole batches with the dummy slot np, so each lane
 // runs its own number of batches with no per-entry test of the list end.
-template <typename real, bool ENERGY, bool DPD = false, bool FLUX = false, bool FORCES = true>
+// MULTI: the instance with several lanes per particle (small systems); the
+// other keeps one lane per particle with no sub-lane bookkeeping.
+template <typename real, bool ENERGY, bool DPD = false, bool FLUX = false, bool FORCES = true, bool MULTI = false>
 __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     using v4 = typename V<real>::v4;
     constexpr bool kGuarded = std::is_same<real, float>::value;
@@ -1172,8 +1172,14 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     }
     __syncthreads();
 
-    const bool valid = static_cast<int>(threadIdx.x) < lb.z;
-    const int i = lb.y + (valid ? static_cast<int>(threadIdx.x) : 0);
+    // P.lpp lanes per particle (a power of two; more than one in small
+    // systems, to fill the device): sub-lane `sub` takes batches sub, sub +
+    // lpp, ... of the list, and the partial sums meet in a fixed butterfly.
+    const int lpp = MULTI ? P.lpp : 1;
+    const int sub = MULTI ? static_cast<int>(threadIdx.x) & (lpp - 1) : 0;
+    const int pl = MULTI ? static_cast<int>(threadIdx.x) >> P.lpp_log2 : static_cast<int>(threadIdx.x);
+    const bool valid = pl < lb.z;
+    const int i = lb.y + (valid ? pl : 0);
     const real cut2 = kGuarded ? real(sd.cutoff2 * (1.0 + kGuard)) : real(sd.cutoff2);
     const real cut2_lo = kGuarded ? real(sd.cutoff2 * (1.0 - kGuard)) : real(sd.cutoff2);
     const bool binside = kGuarded && sd.binside != 0;
@@ -1199,18 +1205,19 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     const long long tstep = sd.time0 + *P.step + P.step_ahead;
     const real dpd_delta = real(sd.inv_sqrt_dt);
     (void)tstep; (void)dpd_delta; (void)S;
-    const int *lp = P.list + list_index(0, i, P.lcap);   // entry k at lp[64 k]
+    const int *lp = P.list + list_index(kListBatch * sub, i, P.lcap);   // entry k at lp[64 (k - 4 sub)]
     // The entries of the next batch are loaded while this batch computes (they
     // stream from HBM); the partner positions of a batch are gathered at its
     // start.  The loads of the next batch are unconditional, so that the wait
     // for a batch's gathers never includes them: after a list's last batch
-    // they read one batch past it, inside the buffer (the allocation has one
-    // batch of slack), and are not used.  Entries are read once per step
+    // they read lpp batches past it, inside the buffer (the allocation has
+    // that slack), and are not used.  Entries are read once per step
     // (non-temporal: L2 stays with the gathers).
     unsigned en[kListBatch];
 #pragma unroll
     for (int u = 0; u < kListBatch; u++) en[u] = static_cast<unsigned>(__builtin_nontemporal_load(lp + 64 * u));
-    for (int k0 = 0; k0 < nbat; k0 += kListBatch) {
+    const int nbat_mine = nbat - kListBatch * sub;   // the batches from this sub-lane's first
+    for (int k0 = 0; k0 < nbat_mine; k0 += kListBatch * lpp) {
         unsigned ev[kListBatch];
         v4 pv[kListBatch];
         v4 vv[DPD ? kListBatch : 1];
@@ -1228,7 +1235,7 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
         // memory counters retire in issue order: the gathers go first, so that
         // waiting for them never waits for the next batch's entries
         __builtin_amdgcn_sched_barrier(0);
-        lp += 64 * kListBatch;
+        lp += 64 * kListBatch * lpp;
 #pragma unroll
         for (int u = 0; u < kListBatch; u++) en[u] = static_cast<unsigned>(__builtin_nontemporal_load(lp + 64 * u));
 #pragma unroll
@@ -1338,7 +1345,22 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
             }
         }
     }
-    if (valid) {
+    // the sub-lanes' partial sums, in a butterfly of fixed order (every
+    // sub-lane ends with the same bits)
+    for (int o = 1; MULTI && o < lpp; o <<= 1) {
+        fx += __shfl_xor(fx, o);
+        fy += __shfl_xor(fy, o);
+        fz += __shfl_xor(fz, o);
+        nd += __shfl_xor(nd, o);
+        if (ENERGY) {
+            ep += __shfl_xor(ep, o);
+            vir += __shfl_xor(vir, o);
+        }
+        if (FLUX)
+#pragma unroll
+            for (int k = 0; k < kMaxSpecies; k++) qacc[k] += __shfl_xor(qacc[k], o);
+    }
+    if (valid && sub == 0) {
         if (FLUX)
 #pragma unroll
             for (int k = 0; k < kMaxSpecies; k++)
@@ -1358,7 +1380,10 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     }
 }
 
-template <typename real, bool DPD, bool FLUX> static const void *list_fn(bool energy) {
+template <typename real, bool DPD, bool FLUX> static const void *list_fn(bool energy, bool multi) {
+    if (multi)
+        return energy ? reinterpret_cast<const void *>(&k_list_force<real, true, DPD, FLUX, true, true>)
+                      : reinterpret_cast<const void *>(&k_list_force<real, false, DPD, FLUX, true, true>);
     return energy ? reinterpret_cast<const void *>(&k_list_force<real, true, DPD, FLUX>)
                   : reinterpret_cast<const void *>(&k_list_force<real, false, DPD, FLUX>);
 }
@@ -1367,9 +1392,9 @@ template <typename real> int run_list_force(DeviceEnsemble *E, const KP<real> &P
     KP<real> Q = P;
     Q.nb = E->nbl;
     void *args[] = {&Q};
-    const bool en = P.energy != 0;
-    const void *fn = E->has_dpd ? (E->has_flux ? list_fn<real, true, true>(en) : list_fn<real, true, false>(en))
-                                : (E->has_flux ? list_fn<real, false, true>(en) : list_fn<real, false, false>(en));
+    const bool en = P.energy != 0, mu = E->lpp > 1;
+    const void *fn = E->has_dpd ? (E->has_flux ? list_fn<real, true, true>(en, mu) : list_fn<real, true, false>(en, mu))
+                                : (E->has_flux ? list_fn<real, false, true>(en, mu) : list_fn<real, false, false>(en, mu));
     const hipError_t e = hipLaunchKernel(fn, dim3(E->nlblocks), dim3(kNbThreads), args, E->nbl.total, E->stream);
     if (e != hipSuccess) return set_error(TFG_E_DEVICE, std::string("list pairs: ") + hipGetErrorString(e));
     return launch_error("list pairs");
@@ -1379,7 +1404,8 @@ template <typename real> int run_list_flux(DeviceEnsemble *E, const KP<real> &P)
     KP<real> Q = P;
     Q.nb = E->nbl;
     void *args[] = {&Q};
-    const void *fn = reinterpret_cast<const void *>(&k_list_force<real, false, false, true, false>);
+    const void *fn = E->lpp > 1 ? reinterpret_cast<const void *>(&k_list_force<real, false, false, true, false, true>)
+                                : reinterpret_cast<const void *>(&k_list_force<real, false, false, true, false>);
     const hipError_t e = hipLaunchKernel(fn, dim3(E->nlblocks), dim3(kNbThreads), args, E->nbl.total, E->stream);
     if (e != hipSuccess) return set_error(TFG_E_DEVICE, std::string("list flux: ") + hipGetErrorString(e));
     return launch_error("list flux");
