@@ -122,9 +122,10 @@ int tfg_engine_set_wall_potential(tfg_engine *e, int face, int type, const tfg_p
  * is the entries per particle (0: from the density); a step that would
  * exceed it fails with TFG_E_RANGE.  Lists are used when every engine of the
  * device copy asks for them, integrates with forward Euler, has tabulated
- * potentials only (no DPD, flux or radius transforms) whose tables fit the
- * LDS cache, and its cutoff + skin spans at most two cells per axis
- * (periodic axes of five cells or more); otherwise the cell pass runs. */
+ * and DPD potentials (with species fluxes, but no radius transforms, sums or
+ * periodic images) whose tables fit the LDS cache, and its cutoff + skin
+ * spans at most two cells per axis and less than half the box along a
+ * periodic axis; otherwise the cell pass runs. */
 int tfg_engine_set_verlet(tfg_engine *e, double skin, int capacity);
 /* Whether the device copy steps with lists, the rebuilds so far and the
  * capacity; zeros when the engine is not on the device. */
