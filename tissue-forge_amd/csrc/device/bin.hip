@@ -103,6 +103,37 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int *cnt, const
     if (blockIdx.x == 0 && threadIdx.x == 0) start[n] = bsum[nb];
 }
 
+// The whole scan in one block, tile after tile with a carry: for grids of a
+// few tiles, where the three-kernel scan would be mostly launches.
+constexpr int kScanSingleMax = 4 * kScanTile;
+__global__ __launch_bounds__(kScanBlock) void k_scan_single(const int *cnt, int *start, int n, const int *gate,
+                                                           const int *halt) {
+    if (bin_skip(gate, halt)) return;
+    __shared__ int s[16];
+    __shared__ int tot;
+    int carry = 0;
+    for (int t0 = 0; t0 < n; t0 += kScanTile) {
+        const int base = t0 + threadIdx.x * kScanPer;
+        int c[kScanPer];
+        int v = 0;
+        for (int k = 0; k < kScanPer; k++) {
+            c[k] = base + k < n ? cnt[base + k] : 0;
+            v += c[k];
+        }
+        const int ex = block_exclusive_scan(v, s);
+        int run = carry + ex;
+        for (int k = 0; k < kScanPer; k++) {
+            if (base + k < n) start[base + k] = run;
+            run += c[k];
+        }
+        if (threadIdx.x == kScanBlock - 1) tot = ex + v;
+        __syncthreads();
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[n] = carry;
+}
+
 // Counts per cell from buffer B (used once after an upload).
 __global__ void k_count(const int4 *meta, int *cnt, int np, const int *gate, const int *halt) {
     if (bin_skip(gate, halt)) return;
@@ -281,9 +312,14 @@ template <typename real> int run_bin(DeviceEnsemble *E, const KP<real> &P) {
     if ((rc = run_sync<real>(E, P, false)) != TFG_OK) return rc;   // a rebuild bins from B
     if (E->list_mode)
         hipLaunchKernelGGL(k_count, dim3(g), dim3(t), 0, s, E->meta[1], E->cnt, E->np, gate, E->d_err);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->ncells, gate, E->d_err);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanBlock), 0, s, E->bsum, nb, gate, E->d_err);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->start, E->ncells, nb, gate, E->d_err);
+    if (E->ncells <= kScanSingleMax) {
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanBlock), 0, s, E->cnt, E->start, E->ncells, gate, E->d_err);
+    } else {
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->ncells, gate, E->d_err);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanBlock), 0, s, E->bsum, nb, gate, E->d_err);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, E->cnt, E->bsum, E->start, E->ncells, nb,
+                           gate, E->d_err);
+    }
     hipLaunchKernelGGL(k_scatter, dim3(g), dim3(t), 0, s, E->meta[1], E->start, E->fill, E->tmp, E->np, gate, E->d_err);
     // grid-stride over groups of four cells; a modest grid keeps the launch
     // cheap in the list steps that skip it
