@@ -209,15 +209,29 @@ __global__ __launch_bounds__(64) void k_cellorder(const int *start, const int2 *
             big &= big - 1;
             const int cb = c0 + gl;
             const int sb = start[cb], nb = start[cb + 1] - sb;
-            const bool lds = nb <= 2048;
+            const bool lds = nb <= 2048 - 8;
             __syncthreads();
-            if (lds)
+            if (lds) {
                 for (int k = lane; k < nb; k += kWave) keys[k] = tmp[sb + k].x;
+                // eight keys past the end that rank below nothing
+                if (lane < 8) keys[nb + lane] = 0x7fffffff;
+            }
             __syncthreads();
             for (int k = lane; k < nb; k += kWave) {
                 const int2 mine = tmp[sb + k];
                 int rk = 0;
-                for (int o = 0; o < nb; o++) rk += (lds ? keys[o] : tmp[sb + o].x) < mine.x;
+                if (lds) {
+                    // eight keys per step, their LDS reads in flight together
+                    for (int o = 0; o < nb; o += 8) {
+                        int kk[8];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) kk[u] = keys[o + u];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) rk += kk[u] < mine.x;
+                    }
+                } else {
+                    for (int o = 0; o < nb; o++) rk += tmp[sb + o].x < mine.x;
+                }
                 perm[sb + rk] = mine.y;
             }
         }
