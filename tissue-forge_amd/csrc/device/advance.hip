@@ -222,8 +222,8 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                 const int kind = ent >> 30;   // 0 bond, 1 angle
                 const int idx = ent & 0x3fffffff;
                 if (kind == 0) {
-                    const int4 b = P.bonds[idx];
-                    const int si = P.slot_of[b.x], sj = P.slot_of[b.y];
+                    const int4 b = P.bslot[idx];   // slots of the bond's particles, potential
+                    const int si = b.x, sj = b.y;
                     const v4 xi = P.A.pos[si], xj = P.A.pos[sj];
                     int li[3], lj[3];
                     cell_loc(P.A.meta[si].z - sd.cell_off, sd, li);
@@ -245,14 +245,14 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                     real e, eff;
                     eval_table_r<real>(pd, P.coef, r, &e, &eff);
                     const real fr = eff / r;
-                    const real sgn = (id == b.x) ? real(-1) : real(1);
+                    const real sgn = (i == si) ? real(-1) : real(1);
                     f.x += sgn * fr * d[0];
                     f.y += sgn * fr * d[1];
                     f.z += sgn * fr * d[2];
-                    if (id == b.x) eb += static_cast<double>(e);
+                    if (i == si) eb += static_cast<double>(e);
                 } else {
-                    const int4 a = P.angles[idx];
-                    const int si = P.slot_of[a.x], sj = P.slot_of[a.y], sk = P.slot_of[a.z];
+                    const int4 a = P.aslot[idx];   // slots of the angle's particles, potential
+                    const int si = a.x, sj = a.y, sk = a.z;
                     const v4 pi = P.A.pos[si], pj = P.A.pos[sj], pk = P.A.pos[sk];
                     int li[3], lj[3], lk[3];
                     cell_loc(P.A.meta[si].z - sd.cell_off, sd, li);
@@ -282,9 +282,9 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
                         dxk[k] = (rji[k] * inji - ct * rjk[k] * injk) * injk;
                     }
                     real g[3];
-                    if (id == a.x) {
+                    if (i == si) {
                         for (int k = 0; k < 3; k++) g[k] = -eff * dxi[k];
-                    } else if (id == a.z) {
+                    } else if (i == sk) {
                         for (int k = 0; k < 3; k++) g[k] = -eff * dxk[k];
                     } else {
                         for (int k = 0; k < 3; k++) g[k] = eff * dxi[k] + eff * dxk[k];

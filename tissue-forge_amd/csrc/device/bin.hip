@@ -134,6 +134,22 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_single(const int *cnt, int 
     if (threadIdx.x == 0) start[n] = carry;
 }
 
+// The slots of every bond's and angle's particles (and its potential), after
+// a reordering: the update then reads them without the id -> slot lookups.
+__global__ void k_term_slots(const int4 *bonds, const int4 *angles, const int *slot_of, int4 *bslot, int4 *aslot,
+                             int nbonds, int nangles, const int *gate, const int *halt) {
+    if (bin_skip(gate, halt)) return;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbonds + nangles; t += gridDim.x * blockDim.x) {
+        if (t < nbonds) {
+            const int4 b = bonds[t];
+            bslot[t] = make_int4(slot_of[b.x], slot_of[b.y], b.z, 0);
+        } else {
+            const int4 a = angles[t - nbonds];
+            aslot[t - nbonds] = make_int4(slot_of[a.x], slot_of[a.y], slot_of[a.z], a.w);
+        }
+    }
+}
+
 // Counts per cell from buffer B (used once after an upload).
 __global__ void k_count(const int4 *meta, int *cnt, int np, const int *gate, const int *halt) {
     if (bin_skip(gate, halt)) return;
@@ -341,6 +357,9 @@ template <typename real> int run_bin(DeviceEnsemble *E, const KP<real> &P) {
     hipLaunchKernelGGL(k_cellorder, dim3(oc), dim3(64), 0, s, E->start, E->tmp, E->perm, E->cnt, E->fill, E->ncells,
                        gate, E->d_err);
     hipLaunchKernelGGL(k_permute<real>, dim3(g), dim3(t), 0, s, P);
+    if (E->nbonds + E->nangles > 0)
+        hipLaunchKernelGGL(k_term_slots, dim3(g), dim3(t), 0, s, E->d_bonds, E->d_angles, E->slot_of, E->d_bslot,
+                           E->d_aslot, E->nbonds, E->nangles, gate, E->d_err);
     return launch_error("binning");
 }
 

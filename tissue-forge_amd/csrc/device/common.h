@@ -171,6 +171,9 @@ template <typename real> struct KP {
     double *e_nb, *e_vir, *e_kin, *e_bond, *e_ang;
     const int *inc_off, *inc;
     const int4 *bonds, *angles;
+    int4 *bslot, *aslot;              // per bond (slot i, slot j, potential) and angle (slots i, j, k,
+                                      // potential), refreshed by the binning whenever slots change
+    int nbonds, nangles;
     long long *step;
     // err[0]: halt word, read by every kernel of a step at its start; bit
     //   kHaltTooFast (folded in from err[1] by the next step's tick) or

@@ -116,6 +116,8 @@ struct DeviceEnsemble {
     double *d_energy = nullptr, *d_spec_tot = nullptr, *sp_part = nullptr;
     int *inc_off = nullptr, *inc = nullptr;
     int4 *d_bonds = nullptr, *d_angles = nullptr;
+    int4 *d_bslot = nullptr, *d_aslot = nullptr;
+    int nbonds = 0, nangles = 0;
     long long *d_step = nullptr;
     int *d_err = nullptr;
 

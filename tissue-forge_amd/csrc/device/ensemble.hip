@@ -62,6 +62,10 @@ template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy) {
     P.inc = E->inc;
     P.bonds = E->d_bonds;
     P.angles = E->d_angles;
+    P.bslot = E->d_bslot;
+    P.aslot = E->d_aslot;
+    P.nbonds = E->nbonds;
+    P.nangles = E->nangles;
     P.step = E->d_step;
     P.err = E->d_err;
     P.ncells = E->ncells;
@@ -660,6 +664,10 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     ALLOC(E->inc, int, inc.size());
     ALLOC(E->d_bonds, int4, hb.size());
     ALLOC(E->d_angles, int4, ha.size());
+    ALLOC(E->d_bslot, int4, hb.size());
+    ALLOC(E->d_aslot, int4, ha.size());
+    E->nbonds = static_cast<int>(hb.size());
+    E->nangles = static_cast<int>(ha.size());
     ALLOC(E->d_lflags, int, 4);
     ALLOC(E->d_sflags, int, 3 * nsys);
     if (E->list_mode) {
