@@ -163,6 +163,99 @@ __device__ void reflect(const SysDesc &sd, const int loc[3], real xv[3], real vv
             delta[k] = (xv[k] >= real(sd.h[k] - sd.foff[k])) - (xv[k] < real(-sd.foff[k]));
 }
 
+// Bonds and angles, one thread per term, as the reference's per-term loops
+// (bond_eval, tfBond.cpp:97; angle_eval, tfAngle.cpp:90): each term is
+// evaluated once and its results stored, and the update adds them
+// to every particle of the term in the particle's incidence order.  A bond
+// stores the force on its first particle (the second takes its negation)
+// and its energy; an angle the forces on its ends i and k (the centre takes
+// minus their sum) and its energy.  Terms out of range store zeros.
+template <typename real> __global__ __launch_bounds__(256) void k_terms(KP<real> P) {
+    using v4 = typename V<real>::v4;
+    if (P.err[0] != 0) return;   // halted: the update does not read the terms
+    const v4 zero = v4_of<real>(real(0), real(0), real(0));
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P.nbonds + P.nangles; t += gridDim.x * blockDim.x) {
+        if (t < P.nbonds) {
+            const int4 b = P.bslot[t];   // slots of the bond's particles, potential
+            const int si = b.x, sj = b.y;
+            const int ci = P.A.meta[si].z, cj = P.A.meta[sj].z;
+            const SysDesc &sd = P.sys[P.cell_sys[ci]];
+            const v4 xi = P.A.pos[si], xj = P.A.pos[sj];
+            int li[3], lj[3];
+            cell_loc(ci - sd.cell_off, sd, li);
+            cell_loc(cj - sd.cell_off, sd, lj);
+            const real h[3] = {real(sd.h[0]), real(sd.h[1]), real(sd.h[2])};
+            const real xiv[3] = {xi.x, xi.y, xi.z}, xjv[3] = {xj.x, xj.y, xj.z};
+            real d[3];
+            for (int k = 0; k < 3; k++) d[k] = (xiv[k] + h[k] * wrap_shift(li[k] - lj[k])) - xjv[k];
+            const real r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const PotDesc<real> &pd = P.pots[b.z];
+            real rr = sqrt(r2);
+            if (pd.flags & POT_SCALED)
+                rr = rr / (xi.w + xj.w);
+            else if (pd.flags & POT_SHIFTED)
+                rr = rr - (xi.w + xj.w) + pd.r0p1;
+            real rr2 = rr * rr;
+            v4 out = zero;
+            if (!(rr2 > pd.b * pd.b)) {
+                rr2 = rr2 > pd.a * pd.a ? rr2 : pd.a * pd.a;
+                const real r = sqrt(rr2);
+                real e, eff;
+                eval_table_r<real>(pd, P.coef, r, &e, &eff);
+                const real fr = eff / r;
+                const real sgn = real(-1);   // the first particle's side
+                out.x = sgn * fr * d[0];
+                out.y = sgn * fr * d[1];
+                out.z = sgn * fr * d[2];
+                out.w = e;
+            }
+            P.tforce[t] = out;
+        } else {
+            const int ta = t - P.nbonds;
+            const int4 a = P.aslot[ta];   // slots of the angle's particles, potential
+            const int si = a.x, sj = a.y, sk = a.z;
+            const int cj = P.A.meta[sj].z;
+            const SysDesc &sd = P.sys[P.cell_sys[cj]];
+            const v4 pi = P.A.pos[si], pj = P.A.pos[sj], pk = P.A.pos[sk];
+            int li[3], lj[3], lk[3];
+            cell_loc(P.A.meta[si].z - sd.cell_off, sd, li);
+            cell_loc(cj - sd.cell_off, sd, lj);
+            cell_loc(P.A.meta[sk].z - sd.cell_off, sd, lk);
+            const real h[3] = {real(sd.h[0]), real(sd.h[1]), real(sd.h[2])};
+            const real piv[3] = {pi.x, pi.y, pi.z}, pjv[3] = {pj.x, pj.y, pj.z}, pkv[3] = {pk.x, pk.y, pk.z};
+            real rji[3], rjk[3];
+            for (int k = 0; k < 3; k++) {
+                const real xi = piv[k] + wrap_shift(li[k] - lj[k]) * h[k];
+                const real xk = pkv[k] + wrap_shift(lk[k] - lj[k]) * h[k];
+                rji[k] = xi - pjv[k];
+                rjk[k] = xk - pjv[k];
+            }
+            const real dprod = rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2];
+            const real inji = real(1) / sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+            const real injk = real(1) / sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+            real ct = dprod * inji * injk;
+            ct = ct < real(1) ? ct : real(1);
+            ct = ct > real(-1) ? ct : real(-1);
+            const PotDesc<real> &pd = P.pots[a.w];
+            v4 gi = zero, gk = zero;
+            if (!(ct > pd.b)) {
+                real e, eff;
+                eval_table_r<real>(pd, P.coef, ct > pd.a ? ct : pd.a, &e, &eff);
+                real dxi[3], dxk[3];
+                for (int k = 0; k < 3; k++) {
+                    dxi[k] = (rjk[k] * injk - ct * rji[k] * inji) * inji;
+                    dxk[k] = (rji[k] * inji - ct * rjk[k] * injk) * injk;
+                }
+                gi = v4_of<real>(-eff * dxi[0], -eff * dxi[1], -eff * dxi[2]);
+                gi.w = e;
+                gk = v4_of<real>(-eff * dxk[0], -eff * dxk[1], -eff * dxk[2]);
+            }
+            P.tforce[P.nbonds + 2 * ta] = gi;
+            P.tforce[P.nbonds + 2 * ta + 1] = gk;
+        }
+    }
+}
+
 // RK4 selects the instance with the Runge-Kutta stages (update 2..5); the
 // other serves updates 0 and 1.
 // LEAN: forward Euler for ensembles without bonds, angles, active walls or
@@ -219,80 +312,37 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
             double eb = 0.0, ea = 0.0;
             for (int q = LEAN ? 0 : P.inc_off[id]; q < (LEAN ? 0 : P.inc_off[id + 1]); q++) {
                 const int ent = P.inc[q];
-                const int kind = ent >> 30;   // 0 bond, 1 angle
-                const int idx = ent & 0x3fffffff;
-                if (kind == 0) {
-                    const int4 b = P.bslot[idx];   // slots of the bond's particles, potential
-                    const int si = b.x, sj = b.y;
-                    const v4 xi = P.A.pos[si], xj = P.A.pos[sj];
-                    int li[3], lj[3];
-                    cell_loc(P.A.meta[si].z - sd.cell_off, sd, li);
-                    cell_loc(P.A.meta[sj].z - sd.cell_off, sd, lj);
-                    const real xiv[3] = {xi.x, xi.y, xi.z}, xjv[3] = {xj.x, xj.y, xj.z};
-                    real d[3];
-                    for (int k = 0; k < 3; k++) d[k] = (xiv[k] + h[k] * wrap_shift(li[k] - lj[k])) - xjv[k];
-                    const real r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                    const PotDesc<real> &pd = P.pots[b.z];
-                    real rr = sqrt(r2);
-                    if (pd.flags & POT_SCALED)
-                        rr = rr / (xi.w + xj.w);
-                    else if (pd.flags & POT_SHIFTED)
-                        rr = rr - (xi.w + xj.w) + pd.r0p1;
-                    real rr2 = rr * rr;
-                    if (rr2 > pd.b * pd.b) continue;
-                    rr2 = rr2 > pd.a * pd.a ? rr2 : pd.a * pd.a;
-                    const real r = sqrt(rr2);
-                    real e, eff;
-                    eval_table_r<real>(pd, P.coef, r, &e, &eff);
-                    const real fr = eff / r;
-                    const real sgn = (i == si) ? real(-1) : real(1);
-                    f.x += sgn * fr * d[0];
-                    f.y += sgn * fr * d[1];
-                    f.z += sgn * fr * d[2];
-                    if (i == si) eb += static_cast<double>(e);
-                } else {
-                    const int4 a = P.aslot[idx];   // slots of the angle's particles, potential
-                    const int si = a.x, sj = a.y, sk = a.z;
-                    const v4 pi = P.A.pos[si], pj = P.A.pos[sj], pk = P.A.pos[sk];
-                    int li[3], lj[3], lk[3];
-                    cell_loc(P.A.meta[si].z - sd.cell_off, sd, li);
-                    cell_loc(P.A.meta[sj].z - sd.cell_off, sd, lj);
-                    cell_loc(P.A.meta[sk].z - sd.cell_off, sd, lk);
-                    const real piv[3] = {pi.x, pi.y, pi.z}, pjv[3] = {pj.x, pj.y, pj.z}, pkv[3] = {pk.x, pk.y, pk.z};
-                    real rji[3], rjk[3];
-                    for (int k = 0; k < 3; k++) {
-                        const real xi = piv[k] + wrap_shift(li[k] - lj[k]) * h[k];
-                        const real xk = pkv[k] + wrap_shift(lk[k] - lj[k]) * h[k];
-                        rji[k] = xi - pjv[k];
-                        rjk[k] = xk - pjv[k];
-                    }
-                    const real dprod = rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2];
-                    const real inji = real(1) / sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
-                    const real injk = real(1) / sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
-                    real ct = dprod * inji * injk;
-                    ct = ct < real(1) ? ct : real(1);
-                    ct = ct > real(-1) ? ct : real(-1);
-                    const PotDesc<real> &pd = P.pots[a.w];
-                    if (ct > pd.b) continue;
-                    real e, eff;
-                    eval_table_r<real>(pd, P.coef, ct > pd.a ? ct : pd.a, &e, &eff);
-                    real dxi[3], dxk[3];
-                    for (int k = 0; k < 3; k++) {
-                        dxi[k] = (rjk[k] * injk - ct * rji[k] * inji) * inji;
-                        dxk[k] = (rji[k] * inji - ct * rjk[k] * injk) * injk;
-                    }
-                    real g[3];
-                    if (i == si) {
-                        for (int k = 0; k < 3; k++) g[k] = -eff * dxi[k];
-                    } else if (i == sk) {
-                        for (int k = 0; k < 3; k++) g[k] = -eff * dxk[k];
+                const int role = (ent >> 28) & 3;   // first, second or third particle of the term
+                const int idx = ent & 0x0fffffff;
+                if (!(ent & (1 << 30))) {   // bond: the first particle takes the stored force
+                    const v4 r = P.tforce[idx];
+                    if (role == 0) {
+                        f.x += r.x;
+                        f.y += r.y;
+                        f.z += r.z;
+                        eb += static_cast<double>(r.w);
                     } else {
-                        for (int k = 0; k < 3; k++) g[k] = eff * dxi[k] + eff * dxk[k];
-                        ea += static_cast<double>(e);
+                        f.x -= r.x;
+                        f.y -= r.y;
+                        f.z -= r.z;
                     }
-                    f.x += g[0];
-                    f.y += g[1];
-                    f.z += g[2];
+                } else {   // angle: the ends take theirs, the centre minus their sum
+                    const v4 gi = P.tforce[P.nbonds + 2 * idx];
+                    const v4 gk = P.tforce[P.nbonds + 2 * idx + 1];
+                    if (role == 0) {
+                        f.x += gi.x;
+                        f.y += gi.y;
+                        f.z += gi.z;
+                    } else if (role == 2) {
+                        f.x += gk.x;
+                        f.y += gk.y;
+                        f.z += gk.z;
+                    } else {
+                        f.x -= gi.x + gk.x;
+                        f.y -= gi.y + gk.y;
+                        f.z -= gi.z + gk.z;
+                        ea += static_cast<double>(gi.w);
+                    }
                 }
             }
             const int ti = info & 0xffff;
@@ -647,6 +697,8 @@ template <typename real> int run_flux_integrate(DeviceEnsemble *E, const KP<real
 }
 
 template <typename real> int run_advance(DeviceEnsemble *E, const KP<real> &P, int update) {
+    if (E->nbonds + E->nangles > 0)
+        hipLaunchKernelGGL(k_terms<real>, dim3(E->stream_grid()), dim3(256), 0, E->stream, P);
     if (update >= 2)
         hipLaunchKernelGGL((k_advance<real, true, false>), dim3(E->stream_grid()), dim3(256), 0, E->stream, P, update);
     else if (E->lean_advance)

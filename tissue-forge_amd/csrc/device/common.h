@@ -174,6 +174,8 @@ template <typename real> struct KP {
     int4 *bslot, *aslot;              // per bond (slot i, slot j, potential) and angle (slots i, j, k,
                                       // potential), refreshed by the binning whenever slots change
     int nbonds, nangles;
+    typename V<real>::v4 *tforce;     // term results: per bond the force on its first particle (w: energy),
+                                      // per angle those on its ends i (w: energy) and k
     long long *step;
     // err[0]: halt word, read by every kernel of a step at its start; bit
     //   kHaltTooFast (folded in from err[1] by the next step's tick) or

@@ -117,6 +117,7 @@ struct DeviceEnsemble {
     int *inc_off = nullptr, *inc = nullptr;
     int4 *d_bonds = nullptr, *d_angles = nullptr;
     int4 *d_bslot = nullptr, *d_aslot = nullptr;
+    void *d_tforce = nullptr;
     int nbonds = 0, nangles = 0;
     long long *d_step = nullptr;
     int *d_err = nullptr;

@@ -66,6 +66,7 @@ template <typename real> KP<real> make_params(DeviceEnsemble *E, int energy) {
     P.aslot = E->d_aslot;
     P.nbonds = E->nbonds;
     P.nangles = E->nangles;
+    P.tforce = static_cast<typename V<real>::v4 *>(E->d_tforce);
     P.step = E->d_step;
     P.err = E->d_err;
     P.ncells = E->ncells;
@@ -588,6 +589,8 @@ template <typename real> static int upload(DeviceEnsemble *E) {
         }
     }
     for (int g = 0; g < np; g++) deg[g + 1] += deg[g];
+    if (hb.size() >= (1u << 28) || ha.size() >= (1u << 28))
+        return set_error(TFG_E_RANGE, "the device module supports fewer than 2^28 bonds and 2^28 angles");
     inc.assign(deg[np], 0);
     // the lean update kernel serves ensembles without bonds, angles, species
     // or active walls
@@ -601,15 +604,17 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     E->inplace = E->list_mode && deg[np] == 0;
     {
         std::vector<int> fillp(deg.begin(), deg.end() - 1);
+        // entry: kind (bit 30: angle), the particle's role in the term
+        // (bits 28-29: first, second, third) and the term's index
         for (size_t b = 0; b < hb.size(); b++) {
             inc[fillp[hb[b].x]++] = static_cast<int>(b);
-            inc[fillp[hb[b].y]++] = static_cast<int>(b);
+            inc[fillp[hb[b].y]++] = (1 << 28) | static_cast<int>(b);
         }
         for (size_t a = 0; a < ha.size(); a++) {
             const int ent = (1 << 30) | static_cast<int>(a);
             inc[fillp[ha[a].x]++] = ent;
-            inc[fillp[ha[a].y]++] = ent;
-            inc[fillp[ha[a].z]++] = ent;
+            inc[fillp[ha[a].y]++] = ent | (1 << 28);
+            inc[fillp[ha[a].z]++] = ent | (2 << 28);
         }
     }
 
@@ -668,6 +673,7 @@ template <typename real> static int upload(DeviceEnsemble *E) {
     ALLOC(E->d_aslot, int4, ha.size());
     E->nbonds = static_cast<int>(hb.size());
     E->nangles = static_cast<int>(ha.size());
+    ALLOC(E->d_tforce, v4, hb.size() + 2 * ha.size());
     ALLOC(E->d_lflags, int, 4);
     ALLOC(E->d_sflags, int, 3 * nsys);
     if (E->list_mode) {
