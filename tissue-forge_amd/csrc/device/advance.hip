@@ -310,38 +310,54 @@ __global__ __launch_bounds__(256) void k_advance(KP<real> P, int update) {
             const real h[3] = {real(sd.h[0]), real(sd.h[1]), real(sd.h[2])};
 
             double eb = 0.0, ea = 0.0;
-            for (int q = LEAN ? 0 : P.inc_off[id]; q < (LEAN ? 0 : P.inc_off[id + 1]); q++) {
-                const int ent = P.inc[q];
-                const int role = (ent >> 28) & 3;   // first, second or third particle of the term
-                const int idx = ent & 0x0fffffff;
-                if (!(ent & (1 << 30))) {   // bond: the first particle takes the stored force
-                    const v4 r = P.tforce[idx];
-                    if (role == 0) {
-                        f.x += r.x;
-                        f.y += r.y;
-                        f.z += r.z;
-                        eb += static_cast<double>(r.w);
-                    } else {
-                        f.x -= r.x;
-                        f.y -= r.y;
-                        f.z -= r.z;
-                    }
-                } else {   // angle: the ends take theirs, the centre minus their sum
-                    const v4 gi = P.tforce[P.nbonds + 2 * idx];
-                    const v4 gk = P.tforce[P.nbonds + 2 * idx + 1];
-                    if (role == 0) {
-                        f.x += gi.x;
-                        f.y += gi.y;
-                        f.z += gi.z;
-                    } else if (role == 2) {
-                        f.x += gk.x;
-                        f.y += gk.y;
-                        f.z += gk.z;
-                    } else {
-                        f.x -= gi.x + gk.x;
-                        f.y -= gi.y + gk.y;
-                        f.z -= gi.z + gk.z;
-                        ea += static_cast<double>(gi.w);
+            // kIncBatch incidence entries at a time: their entries, then their
+            // stored results, loaded together; added in incidence order
+            constexpr int kIncBatch = std::is_same<real, float>::value ? 4 : 2;
+            const int q_end = LEAN ? 0 : P.inc_off[id + 1];
+            for (int q0 = LEAN ? 0 : P.inc_off[id]; q0 < q_end; q0 += kIncBatch) {
+                int ent[kIncBatch];
+                v4 r0[kIncBatch], r1[kIncBatch];
+#pragma unroll
+                for (int u = 0; u < kIncBatch; u++) ent[u] = q0 + u < q_end ? P.inc[q0 + u] : -1;
+#pragma unroll
+                for (int u = 0; u < kIncBatch; u++) {
+                    const int idx = ent[u] & 0x0fffffff;
+                    const bool angle = ent[u] >= 0 && (ent[u] & (1 << 30));
+                    if (ent[u] >= 0) r0[u] = P.tforce[angle ? P.nbonds + 2 * idx : idx];
+                    if (angle) r1[u] = P.tforce[P.nbonds + 2 * idx + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < kIncBatch; u++) {
+                    if (ent[u] < 0) continue;
+                    const int role = (ent[u] >> 28) & 3;   // first, second or third particle of the term
+                    if (!(ent[u] & (1 << 30))) {   // bond: the first particle takes the stored force
+                        const v4 r = r0[u];
+                        if (role == 0) {
+                            f.x += r.x;
+                            f.y += r.y;
+                            f.z += r.z;
+                            eb += static_cast<double>(r.w);
+                        } else {
+                            f.x -= r.x;
+                            f.y -= r.y;
+                            f.z -= r.z;
+                        }
+                    } else {   // angle: the ends take theirs, the centre minus their sum
+                        const v4 gi = r0[u], gk = r1[u];
+                        if (role == 0) {
+                            f.x += gi.x;
+                            f.y += gi.y;
+                            f.z += gi.z;
+                        } else if (role == 2) {
+                            f.x += gk.x;
+                            f.y += gk.y;
+                            f.z += gk.z;
+                        } else {
+                            f.x -= gi.x + gk.x;
+                            f.y -= gi.y + gk.y;
+                            f.z -= gi.z + gk.z;
+                            ea += static_cast<double>(gi.w);
+                        }
                     }
                 }
             }
