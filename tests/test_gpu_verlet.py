@@ -101,11 +101,13 @@ def test_list_small_periodic_boxes(tf, W, O, gpu):
         assert force_errors(fg, oq.particles()["f"]) <= 1e-5
 
 
+@pytest.mark.parametrize("n", [40_000, 100_000])
 @pytest.mark.parametrize("precision,tol", [(1, 1e-10), (0, 1e-5)])
-def test_list_forces(tf, W, O, gpu, precision, tol):
+def test_list_forces(tf, W, O, gpu, precision, tol, n):
     """The forces of a list step (evaluated at the step's start) against the
-    oracle's at the same positions."""
-    scene = W.morse_lj(n=40_000)
+    oracle's at the same positions; 40,000 particles take four lanes each in
+    the list pass, 100,000 two."""
+    scene = W.morse_lj(n=n)
     scene["verlet_skin"] = 0.4
     eng, pots = W.build_engine(scene)
     eng.to_device(precision)
