@@ -106,6 +106,9 @@ __device__ inline double dist2(double4 a, double4 b) {
 
 __device__ inline float fma_of(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ inline double fma_of(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// the 32-bit word holding a value's sign bit (bit 31)
+__device__ inline unsigned sign_word(float v) { return __float_as_uint(v); }
+__device__ inline unsigned sign_word(double v) { return static_cast<unsigned>(static_cast<unsigned long long>(__double_as_longlong(v)) >> 32); }
 
 // Pending-flux accumulators held in registers: element idx (< S) added to
 // through compile-time indices (a runtime index would place the array in
@@ -624,9 +627,22 @@ __device__ __forceinline__ void nonbond_body(const KP<real> &P) {
                         for (int blk = 0; blk < 8; blk++) {
                             if (blk * 16 < my) {
                                 unsigned bits = 0;
+                                if constexpr (BUILD) {
+                                    // the list's reach (which has a relative margin far
+                                    // above rounding): the sign of |d|^2 - reach^2, shifted
+                                    // in from candidate 15 down to candidate 0
 #pragma unroll
-                                for (int t = 0; t < 16; t++)
-                                    bits |= static_cast<unsigned>(dist2(pi4, sp4[blk * 16 + t]) <= cut2) << t;
+                                    for (int t = 15; t >= 0; t--) {
+                                        const v4 q = sp4[blk * 16 + t];
+                                        const real dx = pi4.x - q.x, dy = pi4.y - q.y, dz = pi4.z - q.z;
+                                        const real d = fma_of(dz, dz, fma_of(dy, dy, fma_of(dx, dx, -cut2)));
+                                        bits = __builtin_amdgcn_alignbit(bits, sign_word(d), 31);
+                                    }
+                                } else {
+#pragma unroll
+                                    for (int t = 0; t < 16; t++)
+                                        bits |= static_cast<unsigned>(dist2(pi4, sp4[blk * 16 + t]) <= cut2) << t;
+                                }
                                 const unsigned long long w = static_cast<unsigned long long>(bits) << ((blk & 3) * 16);
                                 if (blk < 4)
                                     m0 |= w;
