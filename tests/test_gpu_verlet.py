@@ -307,11 +307,13 @@ def test_list_dpd_flux_trajectory_fp64(tf, W, O, gpu, fluxsteps):
     assert rel(eg["ekin"], eo["ekin"]) <= 1e-9
 
 
+@pytest.mark.parametrize("n", [20_000, 160_000])
 @pytest.mark.parametrize("precision,tol", [(1, 1e-10), (0, 1e-5)])
-def test_list_dpd_flux_first_step(tf, W, O, gpu, precision, tol):
+def test_list_dpd_flux_first_step(tf, W, O, gpu, precision, tol, n):
     """The first step's DPD forces, number density and species through the
-    lists against the oracle at the positions the path holds."""
-    scene = _dpd_scene(W, n=20000, fluxsteps=2)
+    lists against the oracle at the positions the path holds (160,000
+    particles take one lane each in the list pass, 20,000 several)."""
+    scene = _dpd_scene(W, n=n, fluxsteps=2)
     eng, pots = W.build_engine(scene)
     eng.to_device(precision)
     assert eng.verlet_state()[0]
