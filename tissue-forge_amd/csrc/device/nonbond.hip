@@ -1152,10 +1152,11 @@ __device__ inline double list_guard_r2(const KP<double> &, const SysDesc &, int,
 // partner positions gathered together, then evaluated in list order).  The
 // build pads every list to whole batches with the dummy slot np, so each lane
 // runs its own number of batches with no per-entry test of the list end.
-// MULTI: the instance with several lanes per particle (small systems); the
-// other keeps one lane per particle with no sub-lane bookkeeping.
+// MULTI: the instance with several lanes per particle (small systems, and
+// table-only systems at any size); the other keeps one lane per particle
+// with no sub-lane bookkeeping.
 template <typename real, bool ENERGY, bool DPD = false, bool FLUX = false, bool FORCES = true, bool MULTI = false>
-__global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
+__device__ __forceinline__ void list_force_body(const KP<real> &P) {
     using v4 = typename V<real>::v4;
     constexpr bool kGuarded = std::is_same<real, float>::value;
     constexpr int kCol = type_column<v4>(1);   // bytes per descriptor column
@@ -1396,7 +1397,21 @@ __global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
     }
 }
 
+template <typename real, bool ENERGY, bool DPD = false, bool FLUX = false, bool FORCES = true, bool MULTI = false>
+__global__ __launch_bounds__(kNbThreads) void k_list_force(KP<real> P) {
+    list_force_body<real, ENERGY, DPD, FLUX, FORCES, MULTI>(P);
+}
+
+// The FP32 table pass of every step, held to 80 VGPRs (six waves per SIMD;
+// the template instance needs one more).
+__global__ __launch_bounds__(kNbThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_list_tables_f32(
+    KP<float> P) {
+    list_force_body<float, false, false, false, true, true>(P);
+}
+
 template <typename real, bool DPD, bool FLUX> static const void *list_fn(bool energy, bool multi) {
+    if (std::is_same<real, float>::value && !DPD && !FLUX && multi && !energy)
+        return reinterpret_cast<const void *>(&k_list_tables_f32);
     if (multi)
         return energy ? reinterpret_cast<const void *>(&k_list_force<real, true, DPD, FLUX, true, true>)
                       : reinterpret_cast<const void *>(&k_list_force<real, false, DPD, FLUX, true, true>);
