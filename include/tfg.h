@@ -278,7 +278,8 @@ int tfg_device_set(int ordinal);
 int tfg_device_get(void);
 /* cuda::EngineConfig::clearDevice (source/cuda/tfEngineConfig.cpp:57-65): back to device 0. */
 int tfg_device_clear(void);
-/* Uploads the engine; a second call while on the device is a no-op. */
+/* Uploads the engine; a second call while on the device is a no-op.  An
+ * engine with no particles fails with TFG_E_INVALIDARG. */
 int tfg_engine_to_device(tfg_engine *e, int precision);
 /* Downloads x, v, f, cell ids and species, frees the device copy; a no-op
  * when the engine is not on the device (source/mdcore/src/tfEngine.cpp:1178-1179). */
@@ -340,7 +341,8 @@ int tfg_nonbond_eval(tfg_engine *e);
  * apply) on the null stream; the engine's state is unchanged and the call
  * returns after the GPU work has drained.  Calls on one GPU share the
  * array, so a per-device lock makes overlapping calls from several threads
- * run one after the other.  TFG_E_STATE while the engine is on the device. */
+ * run one after the other.  TFG_E_STATE while the engine is on the device,
+ * TFG_E_INVALIDARG for an engine with no particles. */
 int tfg_nonbond_eval_noalloc(tfg_engine *e, int precision, double *f, double energies[6]);
 
 /* ---- readback (synchronous). ---- */
@@ -370,7 +372,8 @@ int tfg_set_profiling(tfg_engine *e, int on);
 
 /* ---- batched ensemble: independent systems advanced by the same launches
  *      (no reference counterpart: the reference engine is a process-global
- *      singleton, source/mdcore/include/tfEngine.h:1390). ---- */
+ *      singleton, source/mdcore/include/tfEngine.h:1390).  Every member
+ *      needs at least one particle (TFG_E_INVALIDARG otherwise). ---- */
 int tfg_ensemble_create(tfg_ensemble **out, tfg_engine *const *engines, int n, int precision);
 int tfg_ensemble_step(tfg_ensemble *ens, int nsteps, int use_graph);
 int tfg_ensemble_destroy(tfg_ensemble *ens, int download);

@@ -84,6 +84,19 @@ def test_to_device_fails_loudly_without_gpu(tf):
     assert ei.value.code == E_DEVICE
 
 
+def test_engine_without_particles_is_refused(tf):
+    # Checked before any device call, so the same error on a GPU host.
+    eng = tf.Engine(dim=[12, 12, 12], cells=[4, 4, 4], cutoff=3.0)
+    eng.add_type("A")
+    for call in (lambda: eng.to_device(tf.PREC_MIXED), lambda: eng.force_noalloc(tf.PREC_FP64),
+                 lambda: tf.Ensemble([eng], tf.PREC_MIXED)):
+        with pytest.raises(tf.TfgError) as ei:
+            call()
+        assert ei.value.code == E_INVALIDARG
+        assert "no particles" in str(ei.value)
+    assert not eng.on_device
+
+
 def test_geometry_and_insertion(tf):
     eng = tf.Engine(dim=[20.0] * 3, cells=[29] * 3, cutoff=0.6)
     cdim, h, span = eng.geometry()

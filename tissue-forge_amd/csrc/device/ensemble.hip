@@ -226,11 +226,16 @@ template <typename real> static int upload(DeviceEnsemble *E);
 int ensemble_create(DeviceEnsemble **out, Engine *const *engines, int n, int precision) {
     if (n < 1) return set_error(TFG_E_INVALIDARG, "empty ensemble");
     if (precision != PREC_MIXED && precision != PREC_FP64) return set_error(TFG_E_INVALIDARG, "unknown precision");
+    for (int k = 0; k < n; k++) {
+        if (!engines[k]) return set_error(TFG_E_INVALIDARG, "null engine");
+        // A system on the device holds at least one particle; the reference
+        // steps an empty universe as a no-op, the device path refuses it.
+        if (engines[k]->parts.empty()) return set_error(TFG_E_INVALIDARG, "engine has no particles");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return set_error(TFG_E_DEVICE, "no HIP device available");
     for (int k = 0; k < n; k++) {
-        if (!engines[k]) return set_error(TFG_E_INVALIDARG, "null engine");
         if (engines[k]->device) return set_error(TFG_E_STATE, "engine already on a device");
         for (int j = 0; j < k; j++)
             if (engines[j] == engines[k]) return set_error(TFG_E_INVALIDARG, "engine listed twice");
@@ -1034,6 +1039,7 @@ static int force_arena_t(Engine *e, void *arena, size_t cap, double *f, Energies
 int ensemble_force_arena(Engine *e, int precision, void *arena, size_t cap, double *f, Energies *en) {
     if (precision != PREC_MIXED && precision != PREC_FP64) return set_error(TFG_E_INVALIDARG, "unknown precision");
     if (e->device) return set_error(TFG_E_STATE, "engine is on the device; call from_device first");
+    if (e->parts.empty()) return set_error(TFG_E_INVALIDARG, "engine has no particles");
     return precision == PREC_MIXED ? force_arena_t<float>(e, arena, cap, f, en)
                                    : force_arena_t<double>(e, arena, cap, f, en);
 }

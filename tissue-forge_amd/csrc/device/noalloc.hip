@@ -32,6 +32,7 @@ std::mutex g_scratch_lock[kScratchDevices];
 size_t scratch_bytes() { return kScratchBytes; }
 
 int nonbond_eval_no_alloc(Engine *e, int precision, double *f, Energies *en) {
+    if (e->parts.empty()) return set_error(TFG_E_INVALIDARG, "engine has no particles");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(TFG_E_DEVICE, "no HIP device available");
     int dev = -1;
