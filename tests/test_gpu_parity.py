@@ -188,3 +188,40 @@ def test_toggle_round_trip(tf, W, gpu):
     assert np.array_equal(before["species"], after["species"])
     with pytest.raises(tf.TfgError):
         eng.step(1)   # not on the device any more
+
+
+@pytest.mark.parametrize("npart", [1, 2, 3])
+@pytest.mark.parametrize("precision,tol", [(1, 1e-10), (0, 1e-5)])
+def test_tiny_systems(tf, W, O, gpu, npart, precision, tol):
+    """One to three particles in a 3x3x3-cell periodic box (all but a few cells
+    empty, one partly filled wave): forces, potential energy and three
+    Newtonian steps against the oracle; a lone particle feels no force and
+    moves by 3 dt v."""
+    scene = W.morse_lj(n=1000)   # L = 10, 3 cells per axis, cutoff 2.5
+    L = scene["dim"][0]
+    # p0-p1 Morse at r = 1.15, p0-p2 LJ across the x face at r = 1.31,
+    # p1-p2 LJ across the x face at r = 2.41
+    x = np.array([[1.0, 1.0, 1.0], [2.1, 1.3, 0.9], [L - 0.3, 1.2, 1.1]])[:npart]
+    v = np.array([[0.5, -0.25, 0.125], [-0.3, 0.2, 0.1], [0.1, 0.4, -0.2]])[:npart]
+    scene.update(x=x, v=v, ptype=np.array([0, 0, 1], dtype=np.int32)[:npart])
+    for t in scene["types"]:
+        t["dynamics"] = 0
+    pg, po, eg, eo = run_force(tf, W, O, scene, precision)
+    fmax = max(np.abs(po["f"][:, :3]).max(), 1.0)
+    assert np.abs(pg["f"][:, :3] - po["f"][:, :3]).max() <= tol * fmax
+    assert abs(eg["epot"] - eo["epot"]) <= tol * max(abs(eo["epot"]), 1.0)
+    if npart == 1:
+        assert np.abs(pg["f"][:, :3]).max() == 0.0 and eg["epot"] == 0.0
+    else:
+        assert np.abs(po["f"][:, :3]).max() > 1e-3   # the pairs are in range
+    eng, pots = W.build_engine(scene)
+    orc = O.Oracle(scene, pots)
+    eng.to_device(precision)
+    eng.step(3)
+    assert orc.step(3, 1) == 0
+    pg, po = eng.particles(), orc.particles()
+    eng.from_device()
+    assert np.array_equal(pg["cid"], po["cid"])
+    assert np.abs(pg["x"] - po["x"]).max() <= tol * L
+    if npart == 1:
+        assert np.abs(pg["x"][0] - (x[0] + 3 * scene["dt"] * v[0])).max() <= tol * L
