@@ -190,13 +190,14 @@ def test_toggle_round_trip(tf, W, gpu):
         eng.step(1)   # not on the device any more
 
 
+@pytest.mark.parametrize("skin", [0.0, 0.5])
 @pytest.mark.parametrize("npart", [1, 2, 3])
 @pytest.mark.parametrize("precision,tol", [(1, 1e-10), (0, 1e-5)])
-def test_tiny_systems(tf, W, O, gpu, npart, precision, tol):
+def test_tiny_systems(tf, W, O, gpu, npart, precision, tol, skin):
     """One to three particles in a 3x3x3-cell periodic box (all but a few cells
-    empty, one partly filled wave): forces, potential energy and three
-    Newtonian steps against the oracle; a lone particle feels no force and
-    moves by 3 dt v."""
+    empty, one partly filled wave), by the cell pass and with neighbour lists:
+    forces, potential energy and three Newtonian steps against the oracle; a
+    lone particle feels no force and moves by 3 dt v."""
     scene = W.morse_lj(n=1000)   # L = 10, 3 cells per axis, cutoff 2.5
     L = scene["dim"][0]
     # p0-p1 Morse at r = 1.15, p0-p2 LJ across the x face at r = 1.31,
@@ -206,7 +207,15 @@ def test_tiny_systems(tf, W, O, gpu, npart, precision, tol):
     scene.update(x=x, v=v, ptype=np.array([0, 0, 1], dtype=np.int32)[:npart])
     for t in scene["types"]:
         t["dynamics"] = 0
-    pg, po, eg, eo = run_force(tf, W, O, scene, precision)
+    eng, pots = W.build_engine(scene)
+    eng.set_verlet(skin)
+    orc = O.Oracle(scene, pots)
+    orc.force(1)
+    po, eo = orc.particles(), orc.energies()
+    eng.to_device(precision)
+    assert eng.verlet_state()[0] == (skin > 0)
+    eng.force()
+    pg, eg = eng.particles(), eng.energies()
     fmax = max(np.abs(po["f"][:, :3]).max(), 1.0)
     assert np.abs(pg["f"][:, :3] - po["f"][:, :3]).max() <= tol * fmax
     assert abs(eg["epot"] - eo["epot"]) <= tol * max(abs(eo["epot"]), 1.0)
@@ -214,10 +223,8 @@ def test_tiny_systems(tf, W, O, gpu, npart, precision, tol):
         assert np.abs(pg["f"][:, :3]).max() == 0.0 and eg["epot"] == 0.0
     else:
         assert np.abs(po["f"][:, :3]).max() > 1e-3   # the pairs are in range
-    eng, pots = W.build_engine(scene)
-    orc = O.Oracle(scene, pots)
-    eng.to_device(precision)
     eng.step(3)
+    orc = O.Oracle(scene, pots)
     assert orc.step(3, 1) == 0
     pg, po = eng.particles(), orc.particles()
     eng.from_device()
