@@ -75,3 +75,48 @@ def test_ensemble_mixed_forces_match_oracle(tf, W, O, gpu):
                                                                       np.percentile(rel, 99), rel.max()))
         assert d.max() / scale <= 1e-5, k
         assert np.percentile(rel, 99) <= 1e-5, k
+
+
+def _few(W, npart):
+    """One to three Newtonian particles in a 3x3x3-cell periodic box, pairs
+    in range across the x face."""
+    sc = W.morse_lj(n=1000)   # L = 10, cutoff 2.5
+    L = sc["dim"][0]
+    x = np.array([[1.0, 1.0, 1.0], [2.1, 1.3, 0.9], [L - 0.3, 1.2, 1.1]])[:npart]
+    v = np.array([[0.5, -0.25, 0.125], [-0.3, 0.2, 0.1], [0.1, 0.4, -0.2]])[:npart]
+    sc.update(x=x, v=v, ptype=np.array([0, 0, 1], dtype=np.int32)[:npart])
+    for t in sc["types"]:
+        t["dynamics"] = 0
+    return sc
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.5])
+def test_ensemble_with_few_particle_members(tf, W, gpu, skin):
+    """Members of one to a few thousand particles in one ensemble advance as
+    each does alone: bitwise by the cell pass; with lists (whose lane count
+    follows the ensemble's size) to FP32 rounding."""
+    scenes = [_few(W, 1)] + W.ensemble(nrep=2, nmin=1500, nmax=3000) + [_few(W, 3), _few(W, 2)]
+    alone = []
+    for sc in scenes:
+        eng, _ = W.build_engine(sc)
+        eng.set_verlet(skin)
+        eng.to_device(0)
+        eng.step(5)
+        alone.append(eng.particles())
+        eng.from_device()
+    engines = [W.build_engine(sc)[0] for sc in scenes]
+    for e in engines:
+        e.set_verlet(skin)
+    ens = tf.Ensemble(engines, precision=0)
+    ens.step(5)
+    ens.close(download=True)
+    for k, e in enumerate(engines):
+        p = e.particles()
+        assert p["x"].shape == alone[k]["x"].shape
+        if skin == 0:
+            assert np.array_equal(p["cid"], alone[k]["cid"]), k
+            assert np.array_equal(p["x"], alone[k]["x"]), k
+            assert np.array_equal(p["f"], alone[k]["f"]), k
+        else:
+            L = scenes[k]["dim"][0]
+            assert np.abs(p["x"] - alone[k]["x"]).max() <= 1e-5 * L, k
