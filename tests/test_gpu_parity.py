@@ -232,3 +232,61 @@ def test_tiny_systems(tf, W, O, gpu, npart, precision, tol, skin):
     assert np.abs(pg["x"] - po["x"]).max() <= tol * L
     if npart == 1:
         assert np.abs(pg["x"][0] - (x[0] + 3 * scene["dt"] * v[0])).max() <= tol * L
+
+
+def _crowded_cell(W):
+    """2,197 particles on a jittered 0.8 lattice filling one 10.5-wide cell
+    (more than the 2,040 the per-cell id ranking holds in LDS) and 60 spread
+    over other cells; one type, Morse with its table from r = 0.5.  The
+    positions lie on a 2^-10 grid, which FP32 holds exactly in the box frame
+    and in each cell's frame (cell edges are multiples of 10.5), so the mixed
+    check measures arithmetic: at this spacing, on the steep side of the
+    Morse well, rounding the inputs to FP32 alone moves forces by ~1.5e-5 of
+    their rms."""
+    sc = W.morse_lj(n=1000)
+    L, h = 31.5, 10.5
+    rng = np.random.default_rng(21)
+    g = np.arange(13) * 0.8 + 0.05
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    lat += rng.uniform(-0.02, 0.02, lat.shape)
+    far = rng.uniform(h, L, size=(60, 3))
+    x = np.round(np.concatenate([lat, far]) * 1024.0) / 1024.0
+    sc.update(dim=[L] * 3, cells=[3] * 3, x=x, v=np.zeros_like(x), ptype=np.zeros(len(x), dtype=np.int32))
+    return sc
+
+
+@pytest.mark.parametrize("precision,tol", [(1, 1e-10), (0, 1e-5)])
+def test_crowded_cell(tf, W, O, gpu, precision, tol):
+    """A cell of 2,197 particles: forces and energy of the cell pass against
+    the oracle, and (FP64) three steps with the per-cell id order taken by
+    the global-memory ranking."""
+    scene = _crowded_cell(W)
+    pg, po, eg, eo = run_force(tf, W, O, scene, precision)
+    assert np.array_equal(pg["cid"], po["cid"])
+    assert np.sum(po["cid"] == po["cid"][0]) == 2197
+    er = force_errors(pg["f"], po["f"])
+    assert er["err_rms"] <= tol
+    if precision == 1:
+        assert rel(eg["epot"], eo["epot"]) <= tol
+    else:
+        # The total (~158) is the remainder of ~1e5 pair energies of both
+        # signs; FP32 arithmetic is bounded against their absolute sum, here
+        # from the analytic Morse form.
+        x, L, m = scene["x"], scene["dim"][0], scene["pots"][0]
+        d = x[:, None, :] - x[None, :, :]
+        d -= L * np.round(d / L)
+        r = np.sqrt((d ** 2).sum(-1))[np.triu_indices(len(x), 1)]
+        r = r[r <= m["max"]]
+        z = np.exp(-m["a"] * (r - m["r0"]))
+        scale = np.abs(m["d"] * (z * z - 2.0 * z)).sum()
+        assert abs(eg["epot"] - eo["epot"]) <= 1e-6 * scale
+    if precision == 1:
+        eng, pots = W.build_engine(scene)
+        orc = O.Oracle(scene, pots)
+        eng.to_device(1)
+        eng.step(3)
+        assert orc.step(3, 4) == 0
+        pg, po = eng.particles(), orc.particles()
+        eng.from_device()
+        assert np.array_equal(pg["cid"], po["cid"])
+        assert np.abs(pg["x"] - po["x"]).max() <= 1e-9
