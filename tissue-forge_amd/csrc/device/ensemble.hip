@@ -228,8 +228,8 @@ int ensemble_create(DeviceEnsemble **out, Engine *const *engines, int n, int pre
     if (precision != PREC_MIXED && precision != PREC_FP64) return set_error(TFG_E_INVALIDARG, "unknown precision");
     for (int k = 0; k < n; k++) {
         if (!engines[k]) return set_error(TFG_E_INVALIDARG, "null engine");
-        // A system on the device holds at least one particle; the reference
-        // steps an empty universe as a no-op, the device path refuses it.
+        // A system on the device holds at least one particle (the reference's
+        // upload has no such check, tfRunner_cuda.cu:3881-4190).
         if (engines[k]->parts.empty()) return set_error(TFG_E_INVALIDARG, "engine has no particles");
     }
     int ndev = 0;
